@@ -1,0 +1,225 @@
+"""Benchmark: seconds per volume of the helical FDK path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--path NAME]
+
+A step is one complete reconstruction (rebin -> filter -> backproject of all
+planned blocks) of the 512 x 512 x 256 clinical workload from 8192 views of a
+64 x 736 detector that are already resident in HBM.  The headline path is
+the mixed-precision one (f16 filtered projections, FP32 accumulation); the
+plain FP32 path (the reference's per-block loop on the GPU) is timed beside
+it, and the FP64 oracle is timed on the host on a bounded sample.
+
+Prints one JSON line (rank 0).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "s/volume + backproj GUPS, 512²×256: FP32 vs mixed/learned at ≤1 HU RMSE"
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_path(ctx, pl, opts, a, c, fan, vol, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        ctx.reconstruct(pl, opts, a, c, fan, vol)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    stage = {"rebin_ms": 0.0, "filter_ms": 0.0, "backproject_ms": 0.0}
+    launches = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = []
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        ctx.reconstruct(pl, opts, a, c, fan, vol)
+        t = ctx.timings()  # profiling mode synchronises each stage
+        per.append(time.perf_counter() - s0)
+        for k in stage:
+            stage[k] += t[k]
+        launches += t["n_backproject_launches"]
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    updates = t["backproject_updates"]
+    return {
+        "s_per_volume": total / steps,
+        "median_s": sorted(per)[len(per) // 2],
+        "stage_ms": {k: v / steps for k, v in stage.items()},
+        "bp_ms_per_launch": stage["backproject_ms"] / max(launches, 1),
+        "bp_launches_per_volume": launches / steps,
+        "updates": updates,
+    }
+
+
+def cpu_baseline(w, fan_np, a, c, sample_slices=(0, 11, 22)):
+    """FP64 oracle on the host: block 0 (rebin + filter of its whole window, and
+    `sample_slices` backprojected), scaled to a whole volume."""
+    from oracle import oracle as O
+    p = O.Prm()
+    pp = w.prm()
+    for f, _ in O.Prm._fields_:
+        setattr(p, f, getattr(pp, f))
+    pl = O.plan_helical(p, a, c)
+    t0 = time.perf_counter()
+    O.reconstruct(pl, fan_np, a, c, window=w.window, blocks=(0, 1), iz_list=[])
+    t_rf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vol = O.reconstruct(pl, fan_np, a, c, window=w.window, blocks=(0, 1),
+                        iz_list=list(sample_slices))
+    t_all = time.perf_counter() - t0
+    t_slice = max(t_all - t_rf, 0.0) / len(sample_slices)
+    per_volume = pl.n_blocks * (t_rf + 32 * t_slice)
+    return {
+        "value": per_volume,
+        "unit": "s/volume",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "sample": (f"FP64 oracle, block 0 of {pl.n_blocks}: rebin+filter of its "
+                   f"{pl.ctg.n_angle_add}-view window ({t_rf:.2f} s) and {len(sample_slices)} of 32 "
+                   f"slices backprojected ({t_slice:.2f} s/slice), scaled to "
+                   f"{pl.n_blocks} blocks x 32 slices"),
+        "sample_seconds": t_rf + t_all,
+    }, vol, sample_slices
+
+
+def load_traffic(kernel_tag):
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("traffic_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="helical_64x736_512x256")
+    ap.add_argument("--path", default="mixed_f16")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the helical FDK path shards by "
+                                   "z-slab (see DESIGN.md) but only the single-GPU kernel is "
+                                   "built and measured; run with --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from listctrecons_amd import recon, scan
+
+    w = scan.workload(args.workload)
+    ctx = recon.Context(0)
+    log(f"synthesising {w.name}: {w.readings} views of {w.nt}x{w.ns}")
+    fan, a, c = recon.synth_scan(ctx, w)
+    pl = recon.plan_helical(w.prm(), a, c)
+    vol = torch.empty((pl.n_slices_recon, pl.ig.ny, pl.ig.nx), dtype=torch.float32,
+                      device="cuda")
+
+    head_opts = recon.preset(args.path, window=w.window)
+    log(f"timing {args.path}: {args.warmup} warm-up + {args.steps} steps")
+    head = time_path(ctx, pl, head_opts, a, c, fan, vol, args.steps, args.warmup)
+    head_vol = vol.cpu().numpy()
+    log(f"  {head['s_per_volume']*1e3:.2f} ms/volume  stages {head['stage_ms']}")
+
+    plain = None
+    plain_vol = None
+    if not args.no_plain:
+        vol2 = torch.empty_like(vol)
+        log("timing plain FP32 (per-block reference structure)")
+        plain = time_path(ctx, pl, recon.preset("plain", window=w.window), a, c, fan, vol2,
+                          max(1, min(args.steps, 3)), 1)
+        plain_vol = vol2.cpu().numpy()
+        log(f"  {plain['s_per_volume']*1e3:.2f} ms/volume  stages {plain['stage_ms']}")
+
+    storage = {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16"}.get(
+        args.path, "f32")
+    bytes_per_tap = 2 if storage in ("f16", "bf16") else 4
+    updates = head["updates"]
+    bp_s = head["bp_ms_per_launch"] * 1e-3
+    launches = head["bp_launches_per_volume"]
+    alg_bytes = (updates * 2 * 4 * bytes_per_tap + pl.n_slices_recon * pl.ig.nx * pl.ig.ny * 4) \
+        / launches
+    achieved = alg_bytes / bp_s
+    gups = updates / (head["stage_ms"]["backproject_ms"] * 1e-3) / 1e9
+    tag = f"bp_fold_{storage}_{w.name}"
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["s_per_volume"], 6),
+        "unit": "s/volume",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["s_per_volume"] * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "storage": storage,
+        "data": "synthetic: analytic ellipsoid phantom line integrals + Poisson noise",
+        "config": {"workload": w.name, "views": w.readings, "detector": f"{w.nt}x{w.ns}",
+                   "volume": f"{pl.ig.nx}x{pl.ig.ny}x{pl.n_slices_recon}", "path": args.path},
+        "backproject_gups": round(gups, 2),
+        "stage_ms": {k: round(v, 3) for k, v in head["stage_ms"].items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "bp_fold_kernel",
+            "achieved": round(achieved / 1e9, 1),
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": load_traffic(tag),
+            "definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update "
+                          f"+ 4 B per voxel written, per launch; {updates / launches:.4g} updates "
+                          f"per launch",
+        },
+    }
+    if plain:
+        out["fp32_plain_s_per_volume"] = round(plain["s_per_volume"], 6)
+        out["fp32_plain_stage_ms"] = {k: round(v, 3) for k, v in plain["stage_ms"].items()}
+        out["speedup_vs_fp32_plain"] = round(plain["s_per_volume"] / head["s_per_volume"], 3)
+        d = (head_vol - plain_vol).astype(np.float64) * scan.hu_per_unit()
+        out["hu_vs_fp32_plain"] = {"rmse": round(float(np.sqrt(np.mean(d * d))), 4),
+                                   "max_abs": round(float(np.max(np.abs(d))), 3)}
+
+    if not args.no_cpu_baseline:
+        log("timing the FP64 oracle on the host (bounded sample)")
+        fan_np = fan.cpu().numpy()
+        cb, ovol, sl = cpu_baseline(w, fan_np, a, c)
+        out["cpu_baseline"] = cb
+        idx = list(sl)
+        ref = ovol[idx]
+        for name, v in (("mixed", head_vol), ("fp32_plain", plain_vol)):
+            if v is None:
+                continue
+            d = (v[idx] - ref) * scan.hu_per_unit()
+            out[f"hu_vs_fp64_oracle_{name}"] = {
+                "rmse": round(float(np.sqrt(np.mean(d * d))), 4),
+                "max_abs": round(float(np.max(np.abs(d))), 3),
+                "rel_rmse": float(np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref)),
+                "sample": f"block 0 slices {idx}, all voxels"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

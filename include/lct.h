@@ -1,0 +1,209 @@
+/*
+ * lct.h -- C ABI of the MI355X helical cone-beam FDK library (liblct.so).
+ *
+ * This is the boundary below the reference's C++ stage classes: the drop-in
+ * classes of listctrecons_amd/cpp (FileParams, ReconData, Rebiner, Filter,
+ * Backprojector, PreWeighting, FDKReconstructor, ReconstrctorAbstractFactory)
+ * call only these functions, and so do the Python tests and bench.py.
+ * Plain pointers and sizes; no HIP, CUDA or torch types appear here.
+ *
+ * Conventions
+ *  - Every function returns LCT_OK (0) or a negative LCT_ERR_* code; the
+ *    message of the last failure on the calling thread is lct_last_error().
+ *  - d_* arguments are device pointers (from lct_device_malloc or any HIP
+ *    allocation on the context's device); h_* arguments are host pointers.
+ *  - Work is stream-ordered on the context's stream (the null stream unless
+ *    lct_ctx_set_stream was called) and asynchronous; lct_synchronize waits.
+ *  - Array layouts follow the reference: projections [view][row][channel],
+ *    volumes [z][y][x], all row-major, channel / x fastest.
+ */
+#ifndef LCT_H
+#define LCT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define LCT_OK 0
+#define LCT_ERR_INVALID (-1)     /* bad argument or shape */
+#define LCT_ERR_HIP (-2)         /* HIP runtime error (message names the call) */
+#define LCT_ERR_NOMEM (-3)       /* device allocation failed */
+#define LCT_ERR_RANGE (-4)       /* requested reconstruction outside the data */
+#define LCT_ERR_UNSUPPORTED (-5) /* option not available for this geometry */
+
+/* ------------------------------------------------------------- enums */
+enum { LCT_F32 = 0, LCT_F16 = 1, LCT_BF16 = 2 };
+enum { LCT_REBIN_NULL = 0, LCT_REBIN_FAN2PAR = 1, LCT_REBIN_PI_ORI = 2 };
+enum { LCT_WEIGHT_NONE = 0, LCT_WEIGHT_PI_ORI = 1 };
+enum { LCT_WINDOW_NONE = 0, LCT_WINDOW_HANN = 1, LCT_WINDOW_SHEPP_LOGAN = 2 };
+enum { LCT_FILTER_DIRECT_F32 = 0, LCT_FILTER_TOEPLITZ_BF16X3 = 1 };
+enum { LCT_BP_HELICAL = 0, LCT_BP_PI_ORI = 1 };
+enum { LCT_INTERP_LDS = 0, LCT_INTERP_SAMPLER = 1 };
+enum { LCT_PIPE_BLOCKS = 0, LCT_PIPE_RESIDENT = 1 };
+
+/* ------------------------------------------------------------ geometry */
+/* CTGeom, Include/DataStructs.h:9-100 (same fields, same meaning). */
+typedef struct {
+    float orbit, orbit_start;
+    int ns, nt;
+    float ws, wt;
+    int n_angle, n_angle_add;
+    float ds, dt, pitch, source_z0, dso;
+    int n_proj_turn, z_dir, add_projections;
+} lct_ct_geom;
+
+/* ImgGeom, Include/DataStructs.h:102-156. */
+typedef struct {
+    int nx, ny, nz;
+    float wx, wy, wz;
+    float dx, dy, dz;
+} lct_img_geom;
+
+/* The numeric keys of a .prm file, Src/FileParams.cpp:10. */
+typedef struct {
+    int ns, nt;
+    float ds, dt, pitch_value;
+    int nx, ny;
+    float dx, dz, offset_x, offset_y;
+    int readings, view_offset, data_offset;
+    float r_src_to_iso, central_channel;
+    int n_proj_turn;
+} lct_prm;
+
+/* ReconInfo (Include/DataStructs.h:225-273) with the derived geometry. */
+typedef struct {
+    lct_ct_geom ctg;
+    lct_img_geom ig;
+    int n_readings;
+    float data_begin_pos, data_end_pos, allow_begin, allow_end;
+    float start_pos, end_pos, recon_start_pos, recon_end_pos;
+    int n_slices_recon, n_slices_block, n_blocks, data_offset;
+    int recon_direction;
+} lct_plan;
+
+/* ------------------------------------------------------------- context */
+typedef struct lct_ctx lct_ctx;
+
+/* Creates a context on HIP device `device`.  The reference has no explicit
+ * context: its state lives in globals (texture references Rebiner.cu:11,
+ * Backprojector.cu:10) and in CTDataGPU (DataStructs.cpp:57-66). */
+int lct_ctx_create(int device, lct_ctx **out);
+int lct_ctx_destroy(lct_ctx *ctx);
+/* stream: a hipStream_t passed as void* (NULL = null stream). */
+int lct_ctx_set_stream(lct_ctx *ctx, void *stream);
+int lct_synchronize(lct_ctx *ctx);
+const char *lct_last_error(void);
+/* Library version string, e.g. "lct 1.0 gfx950". */
+const char *lct_version(void);
+
+/* Device memory (backs GPUMemoryCtrl<T>, Include/GPUMemoryCtrl.h:6-62). */
+int lct_device_malloc(lct_ctx *ctx, void **d_ptr, size_t bytes);
+int lct_device_free(lct_ctx *ctx, void *d_ptr);
+int lct_memcpy_h2d(lct_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int lct_memcpy_d2h(lct_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int lct_memcpy_d2d(lct_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+int lct_memset_zero(lct_ctx *ctx, void *d_dst, size_t bytes);
+
+/* ------------------------------------------------------------ planner */
+/* ReconData::convert_from_file_params, Src/DataStructs.cpp:121-250, host
+ * only.  h_angles / h_couch hold prm->readings values already past
+ * ViewOffset.  Returns LCT_ERR_RANGE where the reference prints "Requested
+ * reconstruction is outside of allowed data range" and exits. */
+int lct_plan_helical(const lct_prm *prm, const float *h_angles, const float *h_couch,
+                     lct_plan *out);
+/* BlockInfo::update, Src/DataStructs.cpp:77-108 (block 0: :222-238).
+ * ref_view: the view whose table position the block's slice 0 is aligned to
+ * (array_search(blockStart)), for mapping slices to table positions. */
+int lct_block_info(const lct_plan *plan, const float *h_couch, int block, int *proj_start,
+                   int *proj_end, float *block_start, float *block_end, int *ref_view);
+
+/* -------------------------------------------------------- stage kernels
+ * Per-block semantics of the reference stages: one call processes one
+ * 32-slice block on the arrays ReconData holds for it. */
+
+/* RampFilterGPU::calculate_filter, Src/Filter.cpp:8-20 (window NONE gives
+ * the reference taps); Hann / Shepp-Logan apodisation replaces the empty
+ * Include/SmoothWindow.h.  h_taps: 2*ns floats, host. */
+int lct_ramp_taps(int window, int ns, float ds, float *h_taps);
+
+/* Rebiner::rebin, CUDA/Rebiner.cu:13-29 (NULL), :82-103 (FAN2PAR),
+ * :156-179 (PI_ORI), with PiOriPreWeighting (CUDA/PreWeighting.cu:8-29)
+ * fused into the epilogue when weight == LCT_WEIGHT_PI_ORI.
+ * d_fan [n_angle_add][nt][ns] f32 -> d_par [n_angle][nt][ns] f32. */
+int lct_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *g, const float *d_fan,
+              float *d_par);
+
+/* PiOriPreWeighting::weighting, CUDA/PreWeighting.cu:31-44, in place on
+ * d_par [n_angle][nt][ns].  kind LCT_WEIGHT_NONE is NULLPreWeighting. */
+int lct_weighting(lct_ctx *ctx, int kind, const lct_ct_geom *g, float *d_par);
+
+/* Filter::filter, CUDA/Filter.cu:52-73: q[r][is] = sum_k h[is-k+ns] row_r[k]
+ * over rows = n_angle*nt rows of ns channels, in place, FP32. */
+int lct_filter(lct_ctx *ctx, const lct_ct_geom *g, const float *h_taps, float *d_rows,
+               long rows);
+
+/* Backprojector::backproject, CUDA/Backprojector.cu:101-169 (HELICAL) and
+ * :252-307 (PI_ORI): d_q [n_angle][nt][ns] f32, h_betas the block's tube
+ * angles (n_angle + n_proj_turn/2 + 1 values; see the conjugate-index note in
+ * the implementation), d_img [nz][ny][nx] f32. */
+int lct_backproject(lct_ctx *ctx, int kind, const lct_ct_geom *g, const lct_img_geom *ig,
+                    const float *d_q, const float *h_betas, float *d_img);
+
+/* ------------------------------------------------ whole-scan pipeline */
+typedef struct {
+    int pipeline;    /* LCT_PIPE_BLOCKS: the reference's per-block loop
+                        (Src/Reconstructor.cpp:33-48) on device; LCT_PIPE_RESIDENT:
+                        every view rebinned and filtered once, volume
+                        backprojected slab by slab from HBM-resident views */
+    int rebin_kind;  /* LCT_REBIN_* */
+    int weight_kind; /* LCT_WEIGHT_* */
+    int window;      /* LCT_WINDOW_* */
+    int filter_algo; /* LCT_FILTER_* (RESIDENT only; BLOCKS uses DIRECT_F32) */
+    int storage;     /* LCT_F32 / LCT_F16 / LCT_BF16: filtered-projection
+                        storage of the RESIDENT backprojector */
+    int bp_kind;     /* LCT_BP_* */
+    int interp;      /* LCT_INTERP_* (RESIDENT only) */
+} lct_recon_opts;
+
+/* Stage times of the last lct_reconstruct on this context (HIP events). */
+typedef struct {
+    float rebin_ms, filter_ms, backproject_ms, total_ms;
+    int n_backproject_launches;
+    long long backproject_updates; /* voxel-view updates (one per view in a
+                                      voxel's window) */
+} lct_timings;
+
+/* FDKReconstructor::reconstruct (Src/Reconstructor.cpp:30-49) for the whole
+ * planned range.  d_fan: the projection file's views [file_views][nt][ns]
+ * resident in device memory (view 0 = file view 0; the plan's DataOffset is
+ * applied here).  d_vol: [n_slices_recon][ny][nx] f32.  Device scratch is
+ * owned by the context and reused across calls. */
+int lct_reconstruct(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
+                    const float *h_angles, const float *h_couch, const float *d_fan,
+                    long file_views, float *d_vol);
+int lct_get_timings(lct_ctx *ctx, lct_timings *out);
+/* Enables HIP-event timing of every stage launch (off by default). */
+int lct_set_profiling(lct_ctx *ctx, int enable);
+
+/* ------------------------------------------- synthetic scan generator */
+/* Exact line integrals of additive ellipsoids (16 doubles each: centre,
+ * semi-axes, row-major body->world rotation, attenuation) for a curved-
+ * detector helical scan; the source of view v sits at
+ * (-dso sin b, dso cos b, couch[v]), b = angles[v].  Not part of the
+ * reference (it ships no data); it produces every test and benchmark input.
+ * d_out [n_views][nt][ns] f32. */
+int lct_project_ellipsoids(lct_ctx *ctx, const lct_ct_geom *g, const float *h_angles,
+                           const float *h_couch, long view0, long n_views,
+                           const double *h_ell, int n_ell, float *d_out);
+/* In place p -> -ln(max(n,1)/I0), n ~ Poisson(I0 exp(-p)); counter-based,
+ * reproducible for a given seed. */
+int lct_poisson_noise(lct_ctx *ctx, float *d_proj, size_t n, float i0, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LCT_H */

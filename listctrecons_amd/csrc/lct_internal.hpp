@@ -1,0 +1,88 @@
+// lct_internal.hpp -- shared internals of liblct.so (context, errors,
+// device scratch, launch helpers).  Not installed; include/lct.h is the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "lct.h"
+
+#define LCT_PI_D 3.141592653589793238462643383  // Include/Common.h:3
+#define LCT_BLOCK_SLICES 32                     // Include/Common.h:4
+
+struct lct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    lct_timings timings{};
+
+    // Scratch owned by the context, grown on demand and reused across calls.
+    struct Buf {
+        void *ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<Buf> scratch = std::vector<Buf>(8);
+
+    hipEvent_t ev[8] = {};
+    bool events_ready = false;
+};
+
+namespace lct {
+
+// Scratch slots of lct_ctx::scratch.
+enum ScratchSlot {
+    kScratchA = 0,  // rebinned views
+    kScratchB,      // filtered views
+    kScratchTables, // per-view / per-channel geometry tables
+    kScratchTaps,   // filter taps / Toeplitz operands
+    kScratchImg,    // one block image (per-block pipeline)
+    kScratchFan,    // fan window (per-block pipeline)
+    kScratchMisc0,
+    kScratchMisc1,
+};
+
+int set_error(int code, const std::string &msg);
+int hip_error(hipError_t e, const char *what);
+void clear_error();
+
+// Returns a device buffer of at least `bytes` in slot `slot` (grown, never shrunk).
+int scratch(lct_ctx *ctx, int slot, size_t bytes, void **out);
+
+// Host-to-device copy on the context stream; returns after the copy is done.
+int upload(lct_ctx *ctx, void *d, const void *h, size_t bytes);
+
+// Kernel-side launchers (defined in the .hip files).
+int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *g, const float *d_fan,
+                 long fan_view0, long fan_views, float *d_par, long n_out, long out_view0,
+                 int transposed);
+int launch_weighting(lct_ctx *ctx, const lct_ct_geom *g, float *d_par);
+int launch_filter_direct(lct_ctx *ctx, int ns, const float *d_taps, float *d_rows, long rows);
+// Literal per-voxel backprojection of one 32-slice block (tables uploaded here).
+int backproject_block_public(lct_ctx *ctx, int kind, const lct_ct_geom &g,
+                             const lct_img_geom &ig, const float *d_q, const float *h_betas,
+                             float *d_img);
+
+// Host geometry tables evaluated in double from the float32 geometry.
+void host_taps(int window, int ns, float ds, float *h);
+// a_min / a_max / a_middle of each slice of a block (float32, as the reference).
+void bp_windows(const lct_ct_geom &g, const lct_img_geom &ig, int *amin, int *amax, int *amid);
+// cos (first n) and sin (next n) of float32 angles evaluated in double.
+void angle_tables(const float *betas, int n, std::vector<float> &cs);
+
+}  // namespace lct
+
+#define LCT_CHECK_HIP(call)                                   \
+    do {                                                      \
+        hipError_t e__ = (call);                              \
+        if (e__ != hipSuccess) return lct::hip_error(e__, #call); \
+    } while (0)
+
+#define LCT_CHECK(call)               \
+    do {                              \
+        int rc__ = (call);            \
+        if (rc__ != LCT_OK) return rc__; \
+    } while (0)

@@ -1,0 +1,575 @@
+// lct_resident.hip -- the HBM-resident pipeline and its backprojector.
+//
+// Every parallel view the planned blocks need is rebinned once and filtered
+// once into Q, an HBM-resident array in the backprojector's layout:
+//     Q[view][slot_ch][slot_row],  slot_ch = channel + 1  (cp = ns + 3 slots),
+//                                  slot_row = row + 4     (rp = nt + 8 slots, nt % 8 == 0)
+// with zero guard slots, stored as f32, f16 or bf16.  A view is then one
+// contiguous block of cp*rp elements, and the channel window a tile of voxel
+// columns needs is one contiguous range.
+//
+// Backprojector (CUDA/Backprojector.cu:12-99 restructured).  In a voxel's
+// window [a_middle - h, a_middle + h), h = npt/2, every view pair (j, j+h)
+// is visited twice by the reference: once as (ia=j, ia2=j+h) and once as
+// (ia=j+h, ia2=j).  With tube angles beta_{j+h} = beta_j + pi the two visits
+// sample the same two rays with swapped weights, so each pair is evaluated
+// once and counted twice; when both rays leave the detector the two visits
+// pick different rays, and the pair then counts each ray once.  Per
+// (column, pair) the z-independent terms (asin, sqrt, channel taps, source
+// z) are computed once and reused for every slice of the 32-slice block
+// whose window contains the pair.  Geometries where the folding does not
+// hold (odd npt, clipped windows, irregular angles) use the per-voxel
+// kernel of lct_stages.hip on the same resident arrays.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lct_internal.hpp"
+
+namespace lct {
+
+long long count_updates(const lct_plan &plan);
+
+constexpr int kRowPad = 4;  // zero row slots before row 0
+constexpr int kTile = 16;   // 16 x 16 voxel columns per workgroup
+
+template <typename T> struct Cvt;
+template <> struct Cvt<float> {
+    static __device__ __forceinline__ float to(float x) { return x; }
+    static __device__ __forceinline__ float from(float x) { return x; }
+};
+template <> struct Cvt<__half> {
+    static __device__ __forceinline__ __half to(float x) { return __float2half_rn(x); }
+    static __device__ __forceinline__ float from(__half x) { return __half2float(x); }
+};
+template <> struct Cvt<__hip_bfloat16> {
+    static __device__ __forceinline__ __hip_bfloat16 to(float x) { return __float2bfloat16(x); }
+    static __device__ __forceinline__ float from(__hip_bfloat16 x) { return __bfloat162float(x); }
+};
+
+// --------------------------------------------------------------- filter
+
+// Direct FP32 ramp convolution (CUDA/Filter.cu:42-46 order: k ascending,
+// FMA) of 4 consecutive rows of one view, written channel-major into Q.
+template <int J, typename T>
+__global__ __launch_bounds__(256) void filter_to_q_kernel(const float *__restrict__ taps,
+                                                          const float *__restrict__ in,
+                                                          T *__restrict__ q, long n_rows, int ns,
+                                                          int nt, int cp, int rp)
+{
+    constexpr int R = 4;
+    extern __shared__ float sm[];
+    float *st = sm;
+    float *sr = sm + 2 * ns;
+    const long r0 = long(blockIdx.x) * R;
+    for (int i = threadIdx.x; i < 2 * ns; i += blockDim.x) st[i] = taps[i];
+    for (int i = threadIdx.x; i < R * ns; i += blockDim.x) {
+        const long rr = r0 + i / ns;
+        sr[i] = rr < n_rows ? in[size_t(rr) * ns + (i % ns)] : 0.0f;
+    }
+    __syncthreads();
+    const int is0 = threadIdx.x * J;
+    if (is0 >= ns) return;
+    float acc[R][J];
+    float hreg[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        hreg[j] = (is0 + j < ns) ? st[is0 + j + ns] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[r][j] = 0.0f;
+    }
+    for (int k = 0; k < ns; k++) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const float x = sr[r * ns + k];
+#pragma unroll
+            for (int j = 0; j < J; j++) acc[r][j] = fmaf(hreg[j], x, acc[r][j]);
+        }
+#pragma unroll
+        for (int j = J - 1; j > 0; j--) hreg[j] = hreg[j - 1];
+        hreg[0] = st[is0 - k - 1 + ns];
+    }
+    if (r0 >= n_rows) return;
+    const long v = r0 / nt;
+    const int it0 = int(r0 - v * nt);  // multiple of 4 (nt % 8 == 0)
+    T *dst = q + size_t(v) * cp * rp + it0 + kRowPad;
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        if (is0 + j >= ns) break;
+        T *d = dst + size_t(is0 + j + 1) * rp;
+#pragma unroll
+        for (int r = 0; r < R; r++) d[r] = Cvt<T>::to(acc[r][j]);
+    }
+}
+
+template <typename T>
+static int launch_filter_to_q(lct_ctx *ctx, int ns, int nt, const float *d_taps,
+                              const float *d_in, T *d_q, long rows, int cp, int rp)
+{
+    const int J = (ns + 255) / 256;
+    const size_t shm = size_t(2 * ns + 4 * ns) * sizeof(float);
+    const dim3 grid(unsigned((rows + 3) / 4));
+#define LCT_FQ_CASE(JJ)                                                                        \
+    case JJ:                                                                                   \
+        hipLaunchKernelGGL((filter_to_q_kernel<JJ, T>), grid, dim3(256), shm, ctx->stream,     \
+                           d_taps, d_in, d_q, rows, ns, nt, cp, rp);                           \
+        break;
+    switch (J) {
+        LCT_FQ_CASE(1)
+        LCT_FQ_CASE(2)
+        LCT_FQ_CASE(3)
+        LCT_FQ_CASE(4)
+        LCT_FQ_CASE(5)
+        LCT_FQ_CASE(6)
+        LCT_FQ_CASE(7)
+        LCT_FQ_CASE(8)
+    default:
+        return set_error(LCT_ERR_UNSUPPORTED, "filter: ns > 2048 not supported");
+    }
+#undef LCT_FQ_CASE
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
+// ------------------------------------------------------- backprojector
+
+struct FoldBpArgs {
+    const void *q;
+    size_t view_stride;  // cp * rp elements
+    int rp, cp, W, H;    // channel window of W = 2H + 2 slots
+    const float *cosb, *sinb;  // indexed by Q view
+    const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
+    float *vol;
+    int n_tiles_x, n_tiles_y, n_wg, xcd_remap;
+    int nx, ny, h;
+    float wx, wy, wz, dx, dy, dz;
+    float inv_dso, dso_ds, dso2, ws, wt, inv_orbit, z0, dzv, kc, ns_f, nt_f, scale;
+};
+
+template <typename T> struct Chunk {
+    static constexpr int kElems = 16 / int(sizeof(T));  // elements per 16-byte load
+};
+
+// Converts a 16-byte chunk of T to floats in LDS.
+template <typename T>
+__device__ __forceinline__ void chunk_to_lds(float *dst, const uint4 &v)
+{
+    if constexpr (sizeof(T) == 4) {
+        reinterpret_cast<float4 *>(dst)[0] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y),
+                                                         __uint_as_float(v.z), __uint_as_float(v.w));
+    } else {
+        // two 16-bit values per 32-bit word, low half first
+        auto lo = [](unsigned w) { return Cvt<T>::from(__builtin_bit_cast(T, (unsigned short)(w & 0xffffu))); };
+        auto hi = [](unsigned w) { return Cvt<T>::from(__builtin_bit_cast(T, (unsigned short)(w >> 16))); };
+        reinterpret_cast<float4 *>(dst)[0] = make_float4(lo(v.x), hi(v.x), lo(v.y), hi(v.y));
+        reinterpret_cast<float4 *>(dst)[1] = make_float4(lo(v.z), hi(v.z), lo(v.w), hi(v.w));
+    }
+}
+
+// First channel slot of the W-slot window of a view, from the channel
+// coordinate c of the tile centre (window covers every tap of the tile).
+__device__ __forceinline__ int window_start(float c, int H, int cp, int W)
+{
+    int s = int(floorf(c)) + 1 - H;
+    s = s < 0 ? 0 : s;
+    return s > cp - W ? cp - W : s;
+}
+
+template <typename T, int NPRE>
+__global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
+{
+    extern __shared__ float lds[];
+    constexpr int CE = Chunk<T>::kElems;
+    const int W = A.W, rp = A.rp;
+    const int view_elems = W * rp;          // one view's window
+    const int stage = 2 * view_elems;       // views j and j+h
+    const int n_chunks = stage / CE;
+
+    // XCD-aware order: consecutive workgroups are dealt round-robin to the 8
+    // XCDs; give each XCD a contiguous range of tiles so the views it stages
+    // stay in its own L2.
+    int wg = blockIdx.x;
+    if (A.xcd_remap) wg = (wg & 7) * (A.n_wg >> 3) + (wg >> 3);
+    const int tiles = A.n_tiles_x * A.n_tiles_y;
+    const int blk = wg / tiles;
+    const int t = wg - blk * tiles;
+    const int tx0 = (t % A.n_tiles_x) * kTile, ty0 = (t / A.n_tiles_x) * kTile;
+    const int ix = tx0 + int(threadIdx.x & (kTile - 1));
+    const int iy = ty0 + int(threadIdx.x / kTile);
+    const int *st = A.slab_tab + blk * 34;
+    const int qoff = st[0], slice0 = st[1];
+    int am[32];
+#pragma unroll
+    for (int z = 0; z < 32; z++) am[z] = st[2 + z];
+
+    const float xc = (float(ix) - A.wx) * A.dx;
+    const float yc = (float(iy) - A.wy) * A.dy;
+    const float xcc = (float(tx0) + 0.5f * (kTile - 1) - A.wx) * A.dx;
+    const float ycc = (float(ty0) + 0.5f * (kTile - 1) - A.wy) * A.dy;
+    const T *Q = static_cast<const T *>(A.q);
+
+    float acc[32];
+#pragma unroll
+    for (int z = 0; z < 32; z++) acc[z] = 0.0f;
+
+    const int jlo = am[0] - A.h, jhi = am[31];
+
+    uint4 pre[NPRE];
+    auto win_of = [&](int j, int &s1, int &s2) {
+        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
+        const float ph = -xcc * c - ycc * s;
+        const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
+        s1 = window_start(c1, A.H, A.cp, W);
+        s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, W);
+    };
+    auto fetch = [&](int j, int s1, int s2) {
+        const T *va = Q + size_t(qoff + j) * A.view_stride + size_t(s1) * rp;
+        const T *vb = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(s2) * rp;
+#pragma unroll
+        for (int i = 0; i < NPRE; i++) {
+            const int c = int(threadIdx.x) + i * 256;
+            if (c < n_chunks) {
+                const int e = c * CE;
+                const T *src = e < view_elems ? va + e : vb + (e - view_elems);
+                pre[i] = *reinterpret_cast<const uint4 *>(src);
+            }
+        }
+    };
+    auto commit = [&](float *buf) {
+#pragma unroll
+        for (int i = 0; i < NPRE; i++) {
+            const int c = int(threadIdx.x) + i * 256;
+            if (c < n_chunks) chunk_to_lds<T>(buf + c * CE, pre[i]);
+        }
+    };
+
+    int s1n, s2n;
+    win_of(jlo, s1n, s2n);
+    fetch(jlo, s1n, s2n);
+    commit(lds);
+    __syncthreads();
+    int cur = 0;
+    for (int j = jlo; j < jhi; j++) {
+        const int s1 = s1n, s2 = s2n;
+        if (j + 1 < jhi) {
+            win_of(j + 1, s1n, s2n);
+            fetch(j + 1, s1n, s2n);
+        }
+        const float *buf = lds + cur * stage;
+        // z-independent terms of the pair (Backprojector.cu:53-62)
+        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
+        const float c2 = A.cosb[qoff + j + A.h], s2v = A.sinb[qoff + j + A.h];
+        const float phat = -xc * c - yc * s;
+        const float theta = asinf(phat * A.inv_dso);
+        const float delta = theta * A.dso_ds;
+        const float p1 = fminf(fmaxf(A.ws + delta, -1.0f), A.ns_f);
+        const float p2 = fminf(fmaxf(A.ws - delta, -1.0f), A.ns_f);
+        const float fp1 = floorf(p1), fp2 = floorf(p2);
+        const float ap1 = p1 - fp1, ap2 = p2 - fp2;
+        // local channel slots; the window covers every tap of the tile, the
+        // clamp only keeps LDS addressing inside the stage buffer
+        const int l1 = min(max(int(fp1) + 1 - s1, 0), W - 2);
+        const int l2 = min(max(int(fp2) + 1 - s2, 0), W - 2);
+        const float *b1 = buf + l1 * rp;
+        const float *b2 = buf + view_elems + l2 * rp;
+        const float root = sqrtf(A.dso2 - phat * phat);
+        const float lhat = root + xc * s - yc * c;
+        const float lhat2 = root + xc * s2v - yc * c2;
+        const float tz = theta * A.inv_orbit;
+        const float zr = A.z0 + (float(j) + tz) * A.dzv;
+        const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
+        const float k1 = A.kc / lhat, k2 = A.kc / lhat2;
+        const float o1 = -zr * k1, o2 = -zr2 * k2;
+#pragma unroll
+        for (int z = 0; z < 32; z++) {
+            if (am[z] > j && am[z] <= j + A.h) {  // pair inside this slice's window
+                const float zc = (float(z) - A.wz) * A.dz;
+                const float qh = fmaf(zc, k1, o1);
+                const float qh2 = fmaf(zc, k2, o2);
+                const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
+                const float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
+                float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
+                float w2 = out1 ? (out2 ? 0.5f : 1.0f) : (out2 ? 0.0f : 1.0f - wb);
+                const float r1 = fminf(fmaxf(qh + A.wt, -1.0f), A.nt_f);
+                const float r2 = fminf(fmaxf(qh2 + A.wt, -1.0f), A.nt_f);
+                const float fr1 = floorf(r1), fr2 = floorf(r2);
+                const float ar1 = r1 - fr1, ar2 = r2 - fr2;
+                const int i1 = int(fr1) + kRowPad, i2 = int(fr2) + kRowPad;
+                const float a00 = b1[i1], a01 = b1[i1 + 1], a10 = b1[rp + i1], a11 = b1[rp + i1 + 1];
+                const float c00 = b2[i2], c01 = b2[i2 + 1], c10 = b2[rp + i2], c11 = b2[rp + i2 + 1];
+                const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
+                const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
+                const float t1 = fmaf(ar1, ha - la, la);
+                const float t2 = fmaf(ar2, hc - lc, lc);
+                acc[z] = fmaf(w1, t1, fmaf(w2, t2, acc[z]));
+            }
+        }
+        if (j + 1 < jhi) commit(lds + (cur ^ 1) * stage);
+        __syncthreads();
+        cur ^= 1;
+    }
+    if (ix < A.nx && iy < A.ny) {
+        float *o = A.vol + (size_t(slice0) * A.ny + iy) * A.nx + ix;
+        const size_t plane = size_t(A.nx) * A.ny;
+#pragma unroll
+        for (int z = 0; z < 32; z++) o[size_t(z) * plane] = acc[z] * A.scale;
+    }
+}
+
+template <typename T>
+static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
+{
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
+    constexpr int CE = Chunk<T>::kElems;
+    const int chunks = 2 * a.W * a.rp / CE;
+    const size_t shm = size_t(2) * 2 * a.W * a.rp * sizeof(float);
+    if (shm > 160 * 1024)
+        return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
+    const int npre = (chunks + 255) / 256;
+#define LCT_BP_CASE(N)                                                                      \
+    case N:                                                                                 \
+        if (shm > 64 * 1024)                                                                \
+            LCT_CHECK_HIP(hipFuncSetAttribute(                                              \
+                reinterpret_cast<const void *>(&bp_fold_kernel<T, N>),                      \
+                hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));                     \
+        hipLaunchKernelGGL((bp_fold_kernel<T, N>), dim3(unsigned(a.n_wg)), dim3(256), shm,  \
+                           ctx->stream, a);                                                 \
+        break;
+    switch (npre) {
+        LCT_BP_CASE(1)
+        LCT_BP_CASE(2)
+        LCT_BP_CASE(3)
+        LCT_BP_CASE(4)
+        LCT_BP_CASE(5)
+        LCT_BP_CASE(6)
+        LCT_BP_CASE(7)
+        LCT_BP_CASE(8)
+    default:
+        return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window too large");
+    }
+#undef LCT_BP_CASE
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
+// Half-width (in channels) of the channel footprint of a 16x16 tile, with
+// the largest in-plane radius of the volume bounding the asin slope.
+static int tile_half_width(const lct_ct_geom &g, const lct_img_geom &ig)
+{
+    const double rx = (std::max(std::fabs(0.0 - ig.wx), std::fabs(ig.nx - 1.0 - ig.wx)) + 1.0) * ig.dx;
+    const double ry = (std::max(std::fabs(0.0 - ig.wy), std::fabs(ig.ny - 1.0 - ig.wy)) + 1.0) * ig.dy;
+    double u = std::sqrt(rx * rx + ry * ry) / g.dso;
+    u = std::min(u, 0.999);
+    const double slope = 1.0 / std::sqrt(1.0 - u * u);
+    const double half_extent = 0.5 * (kTile - 1) * (ig.dx + ig.dy);  // |dx cos| + |dy sin| bound
+    return int(std::ceil(half_extent * slope / g.ds)) + 2;
+}
+
+struct Resident {
+    long g0 = 0, nviews = 0;
+    std::vector<int> bps;
+};
+
+static int plan_resident(const lct_plan &plan, const float *h_couch, long file_views,
+                         Resident &r)
+{
+    const lct_ct_geom &g = plan.ctg;
+    r.bps.resize(plan.n_blocks);
+    long lo = 1L << 40, hi = -(1L << 40);
+    for (int b = 0; b < plan.n_blocks; b++) {
+        int s = 0;
+        LCT_CHECK(lct_block_info(&plan, h_couch, b, &s, nullptr, nullptr, nullptr, nullptr));
+        r.bps[b] = s;
+        if (s < 0 || long(s) + plan.data_offset < 0 ||
+            long(s) + plan.data_offset + g.n_angle_add > file_views ||
+            s + g.add_projections + g.n_angle > plan.n_readings)
+            return set_error(LCT_ERR_RANGE, "block " + std::to_string(b) +
+                                                 " projection window lies outside the data");
+        lo = std::min(lo, long(s) + g.add_projections);
+        hi = std::max(hi, long(s) + g.add_projections + g.n_angle);
+    }
+    r.g0 = lo;
+    r.nviews = hi - lo;
+    return LCT_OK;
+}
+
+int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opts &o,
+                         const float *h_angles, const float *h_couch, const float *d_fan,
+                         long file_views, float *d_vol)
+{
+    const lct_ct_geom &g = plan.ctg;
+    const lct_img_geom &ig = plan.ig;
+    Resident R;
+    LCT_CHECK(plan_resident(plan, h_couch, file_views, R));
+    const size_t view = size_t(g.nt) * g.ns;
+    const int h = g.n_proj_turn / 2;
+
+    // Can the pair-folded backprojector be used?
+    int amin[32], amax[32], amid[32];
+    bp_windows(g, ig, amin, amax, amid);
+    bool fold = (g.n_proj_turn % 2 == 0) && o.bp_kind == LCT_BP_HELICAL && (g.nt % 8 == 0) &&
+                o.interp == LCT_INTERP_LDS;
+    for (int z = 0; z < 32 && fold; z++)
+        fold = (amid[z] - h >= 0) && (amid[z] + h < g.n_angle) && (z == 0 || amid[z] >= amid[z - 1]);
+    for (long v = R.g0; v + h < R.g0 + R.nviews && fold; v++) {
+        double d = double(h_angles[v + h]) - double(h_angles[v]) - LCT_PI_D;
+        d = std::remainder(d, 2.0 * LCT_PI_D);
+        fold = std::fabs(d) < 2e-5;
+    }
+    if (!fold && o.storage != LCT_F32)
+        return set_error(LCT_ERR_UNSUPPORTED,
+                         "reduced-precision storage needs the pair-folded backprojector (even "
+                         "npt, unclipped windows, half-turn-symmetric angles, nt % 8 == 0)");
+
+    std::vector<float> taps(2 * size_t(g.ns));
+    host_taps(o.window, g.ns, g.ds, taps.data());
+    void *d_taps = nullptr, *d_p = nullptr, *d_q = nullptr, *d_tab = nullptr;
+    LCT_CHECK(scratch(ctx, kScratchTaps, taps.size() * 4, &d_taps));
+    LCT_CHECK(upload(ctx, d_taps, taps.data(), taps.size() * 4));
+    LCT_CHECK(scratch(ctx, kScratchA, view * R.nviews * 4, &d_p));
+    float *P = static_cast<float *>(d_p);
+
+    lct_timings &t = ctx->timings;
+    hipEvent_t e0 = ctx->ev[6], e1 = ctx->ev[7];
+    auto tic = [&]() -> int {
+        if (ctx->profiling) LCT_CHECK_HIP(hipEventRecord(e0, ctx->stream));
+        return LCT_OK;
+    };
+    auto toc = [&](float *acc) -> int {
+        if (!ctx->profiling) return LCT_OK;
+        LCT_CHECK_HIP(hipEventRecord(e1, ctx->stream));
+        LCT_CHECK_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        LCT_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *acc += ms;
+        return LCT_OK;
+    };
+
+    // 1. rebin every needed view once (fused pre-weighting).
+    LCT_CHECK(tic());
+    LCT_CHECK(launch_rebin(ctx, o.rebin_kind, o.weight_kind, &g, d_fan, 0, file_views, P,
+                           R.nviews, R.g0 + plan.data_offset, 0));
+    LCT_CHECK(toc(&t.rebin_ms));
+
+    // Per-view cos/sin (double) of the resident views, plus the block table.
+    const int n_cs = int(R.nviews + h + 1);
+    std::vector<float> cs_host(2 * size_t(n_cs));
+    {
+        std::vector<float> b(n_cs);
+        for (int i = 0; i < n_cs; i++) {
+            const long src = R.g0 + i;
+            b[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
+        }
+        angle_tables(b.data(), n_cs, cs_host);
+    }
+
+    if (!fold) {
+        // Filter in place and backproject each block from its offset with the
+        // per-voxel kernel.
+        LCT_CHECK(tic());
+        LCT_CHECK(launch_filter_direct(ctx, g.ns, static_cast<const float *>(d_taps), P,
+                                       long(R.nviews) * g.nt));
+        LCT_CHECK(toc(&t.filter_ms));
+        const size_t slab = size_t(ig.nx) * ig.ny * ig.nz;
+        std::vector<float> betas(g.n_angle + h + 1);
+        for (int b = 0; b < plan.n_blocks; b++) {
+            const long off = long(R.bps[b]) + g.add_projections - R.g0;
+            for (size_t i = 0; i < betas.size(); i++) {
+                const long src = long(R.bps[b]) + g.add_projections + long(i);
+                betas[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
+            }
+            LCT_CHECK(tic());
+            LCT_CHECK(backproject_block_public(ctx, o.bp_kind, g, ig, P + size_t(off) * view,
+                                               betas.data(), d_vol + size_t(b) * slab));
+            LCT_CHECK(toc(&t.backproject_ms));
+            t.n_backproject_launches++;
+        }
+        return LCT_OK;
+    }
+
+    // 2. filter into Q (backprojector layout, storage precision).
+    const int cp = g.ns + 3, rp = g.nt + 8;
+    const size_t esz = o.storage == LCT_F32 ? 4 : 2;
+    const size_t q_bytes = size_t(cp) * rp * esz * R.nviews;
+    LCT_CHECK(scratch(ctx, kScratchB, q_bytes, &d_q));
+    LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_bytes, ctx->stream));
+    LCT_CHECK(tic());
+    const long rows = long(R.nviews) * g.nt;
+    if (o.storage == LCT_F32)
+        LCT_CHECK(launch_filter_to_q<float>(ctx, g.ns, g.nt, static_cast<const float *>(d_taps),
+                                            P, static_cast<float *>(d_q), rows, cp, rp));
+    else if (o.storage == LCT_F16)
+        LCT_CHECK(launch_filter_to_q<__half>(ctx, g.ns, g.nt, static_cast<const float *>(d_taps),
+                                             P, static_cast<__half *>(d_q), rows, cp, rp));
+    else
+        LCT_CHECK(launch_filter_to_q<__hip_bfloat16>(ctx, g.ns, g.nt,
+                                                     static_cast<const float *>(d_taps), P,
+                                                     static_cast<__hip_bfloat16 *>(d_q), rows,
+                                                     cp, rp));
+    LCT_CHECK(toc(&t.filter_ms));
+
+    // 3. backproject every block from Q in one launch.
+    std::vector<int> tab(34 * size_t(plan.n_blocks));
+    for (int b = 0; b < plan.n_blocks; b++) {
+        tab[34 * b] = int(long(R.bps[b]) + g.add_projections - R.g0);
+        tab[34 * b + 1] = b * LCT_BLOCK_SLICES;
+        for (int z = 0; z < 32; z++) tab[34 * b + 2 + z] = amid[z];
+    }
+    const size_t tab_bytes = cs_host.size() * 4 + tab.size() * 4;
+    LCT_CHECK(scratch(ctx, kScratchTables, tab_bytes, &d_tab));
+    float *d_cs = static_cast<float *>(d_tab);
+    int *d_st = reinterpret_cast<int *>(d_cs + cs_host.size());
+    LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
+    LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
+
+    FoldBpArgs a{};
+    a.q = d_q;
+    a.view_stride = size_t(cp) * rp;
+    a.rp = rp;
+    a.cp = cp;
+    a.H = tile_half_width(g, ig);
+    a.W = std::min(2 * a.H + 2, cp);
+    a.cosb = d_cs;
+    a.sinb = d_cs + n_cs;
+    a.slab_tab = d_st;
+    a.vol = d_vol;
+    a.n_tiles_x = (ig.nx + kTile - 1) / kTile;
+    a.n_tiles_y = (ig.ny + kTile - 1) / kTile;
+    a.nx = ig.nx;
+    a.ny = ig.ny;
+    a.h = h;
+    a.wx = ig.wx;
+    a.wy = ig.wy;
+    a.wz = ig.wz;
+    a.dx = ig.dx;
+    a.dy = ig.dy;
+    a.dz = ig.dz;
+    a.inv_dso = float(1.0 / double(g.dso));
+    a.dso_ds = float(double(g.dso) / double(g.ds));
+    a.dso2 = float(double(g.dso) * double(g.dso));
+    a.ws = g.ws;
+    a.wt = g.wt;
+    a.inv_orbit = float(1.0 / double(g.orbit));
+    a.z0 = g.source_z0;
+    a.dzv = float(double(g.pitch) / double(g.n_proj_turn));
+    a.kc = float(double(g.dso) * double(g.z_dir) / double(g.dt));
+    a.ns_f = float(g.ns);
+    a.nt_f = float(g.nt);
+    a.scale = float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
+    LCT_CHECK(tic());
+    if (o.storage == LCT_F32)
+        LCT_CHECK(launch_bp_fold<float>(ctx, a, plan.n_blocks));
+    else if (o.storage == LCT_F16)
+        LCT_CHECK(launch_bp_fold<__half>(ctx, a, plan.n_blocks));
+    else
+        LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, a, plan.n_blocks));
+    LCT_CHECK(toc(&t.backproject_ms));
+    t.n_backproject_launches++;
+    return LCT_OK;
+}
+
+}  // namespace lct

@@ -1,0 +1,225 @@
+"""Python host API over the C ABI (include/lct.h).
+
+Mirrors the reference's plug-in structure (Include/Rebiner.h, Filter.h,
+Backprojector.h, Reconstructor.h): stage calls with the reference's per-block
+semantics, and ``Context.reconstruct`` for the whole FDK loop of
+Src/Reconstructor.cpp:30-49.  Device memory is torch CUDA tensors (plumbing);
+every computation runs in liblct.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dev(t) -> C.c_void_p:
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous CUDA tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------ host planner
+
+def plan_helical(prm: _lib.Prm, angles: np.ndarray, couch: np.ndarray) -> _lib.Plan:
+    """ReconData::convert_from_file_params (Src/DataStructs.cpp:121-250)."""
+    a = np.ascontiguousarray(angles, np.float32)
+    c = np.ascontiguousarray(couch, np.float32)
+    if a.size < prm.readings or c.size < prm.readings:
+        raise ValueError("angle / couch arrays shorter than Readings")
+    out = _lib.Plan()
+    check(lib().lct_plan_helical(C.byref(prm), _fptr(a), _fptr(c), C.byref(out)))
+    return out
+
+
+def block_info(plan: _lib.Plan, couch: np.ndarray, block: int):
+    """BlockInfo::update (Src/DataStructs.cpp:77-108) ->
+    (proj_start, proj_end, block_start, block_end, ref_view)."""
+    c = np.ascontiguousarray(couch, np.float32)
+    s, e, r = C.c_int(), C.c_int(), C.c_int()
+    bs, be = C.c_float(), C.c_float()
+    check(lib().lct_block_info(C.byref(plan), _fptr(c), block, C.byref(s), C.byref(e),
+                               C.byref(bs), C.byref(be), C.byref(r)))
+    return s.value, e.value, bs.value, be.value, r.value
+
+
+def ramp_taps(window: int, ns: int, ds: float) -> np.ndarray:
+    """RampFilterGPU::calculate_filter (Src/Filter.cpp:8-20) + apodisation."""
+    h = np.empty(2 * ns, np.float32)
+    check(lib().lct_ramp_taps(window, ns, C.c_float(ds), _fptr(h)))
+    return h
+
+
+def block_betas(plan: _lib.Plan, angles: np.ndarray, proj_start: int) -> np.ndarray:
+    """betas of a block (CUDA/Backprojector.cu:145) padded to n_angle + npt/2 + 1."""
+    g = plan.ctg
+    n = g.n_angle + g.n_proj_turn // 2 + 1
+    out = np.zeros(n, np.float32)
+    src = proj_start + g.add_projections
+    avail = max(0, min(n, angles.size - src))
+    out[:avail] = angles[src:src + avail]
+    return out
+
+
+# ------------------------------------------------------------ options
+
+@dataclasses.dataclass
+class Options:
+    pipeline: int = _lib.PIPE_BLOCKS
+    rebin_kind: int = _lib.REBIN_FAN2PAR
+    weight_kind: int = _lib.WEIGHT_NONE
+    window: int = _lib.WINDOW_NONE
+    filter_algo: int = _lib.FILTER_DIRECT_F32
+    storage: int = _lib.F32
+    bp_kind: int = _lib.BP_HELICAL
+    interp: int = _lib.INTERP_LDS
+
+    def to_c(self) -> _lib.ReconOpts:
+        o = _lib.ReconOpts()
+        for f in dataclasses.fields(self):
+            setattr(o, f.name, int(getattr(self, f.name)))
+        return o
+
+
+def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
+    """Named paths: "plain" is the reference's per-block FP32 loop on the GPU;
+    "fast_f32" / "mixed_f16" / "mixed_bf16" are the HBM-resident paths with
+    f32 / f16 / bf16 filtered-projection storage (FP32 accumulation)."""
+    if name == "plain":
+        return Options(window=window)
+    if name == "fast_f32":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window)
+    if name == "mixed_f16":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16,
+                       filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
+    if name == "mixed_bf16":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.BF16,
+                       filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
+    if name == "resident_direct_f16":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16)
+    raise KeyError(name)
+
+
+# ------------------------------------------------------------ context
+
+class Context:
+    """One lct_ctx on one device (CTDataGPU's role, Src/DataStructs.cpp:57-66)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(lib().lct_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.h:
+            lib().lct_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self) -> None:
+        check(lib().lct_synchronize(self.h))
+
+    def set_profiling(self, on: bool) -> None:
+        check(lib().lct_set_profiling(self.h, 1 if on else 0))
+
+    def timings(self) -> dict:
+        t = _lib.Timings()
+        check(lib().lct_get_timings(self.h, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in _lib.Timings._fields_}
+
+    # -------------------------------------------------------- stages
+    def rebin(self, kind: int, weight: int, g: _lib.CTGeom, fan, par) -> None:
+        check(lib().lct_rebin(self.h, kind, weight, C.byref(g), _dev(fan), _dev(par)))
+
+    def weighting(self, kind: int, g: _lib.CTGeom, par) -> None:
+        check(lib().lct_weighting(self.h, kind, C.byref(g), _dev(par)))
+
+    def filter(self, g: _lib.CTGeom, taps: np.ndarray, rows, n_rows: int) -> None:
+        t = np.ascontiguousarray(taps, np.float32)
+        check(lib().lct_filter(self.h, C.byref(g), _fptr(t), _dev(rows), n_rows))
+
+    def backproject(self, kind: int, g: _lib.CTGeom, ig: _lib.ImgGeom, q, betas, img) -> None:
+        b = np.ascontiguousarray(betas, np.float32)
+        if b.size < g.n_angle + g.n_proj_turn // 2 + 1:
+            raise ValueError("betas needs n_angle + n_proj_turn/2 + 1 values")
+        check(lib().lct_backproject(self.h, kind, C.byref(g), C.byref(ig), _dev(q), _fptr(b),
+                                    _dev(img)))
+
+    # -------------------------------------------------------- pipeline
+    def reconstruct(self, plan: _lib.Plan, opts: Options, angles: np.ndarray,
+                    couch: np.ndarray, fan, vol=None):
+        """FDKReconstructor::reconstruct over all planned blocks; fan is the
+        whole projection file on the device ([views][nt][ns] float32)."""
+        import torch
+        a = np.ascontiguousarray(angles, np.float32)
+        c = np.ascontiguousarray(couch, np.float32)
+        g = plan.ctg
+        views = fan.numel() // (g.nt * g.ns)
+        if vol is None:
+            vol = torch.empty((plan.n_slices_recon, plan.ig.ny, plan.ig.nx), dtype=torch.float32,
+                              device=fan.device)
+        oc = opts.to_c()
+        check(lib().lct_reconstruct(self.h, C.byref(plan), C.byref(oc), _fptr(a), _fptr(c),
+                                    _dev(fan), views, _dev(vol)))
+        return vol
+
+    # -------------------------------------------------------- synthetic data
+    def project_ellipsoids(self, g: _lib.CTGeom, angles, couch, ell: np.ndarray, out,
+                           view0: int = 0, n_views: int | None = None) -> None:
+        a = np.ascontiguousarray(angles, np.float32)
+        c = np.ascontiguousarray(couch, np.float32)
+        e = np.ascontiguousarray(ell, np.float64).reshape(-1, 16)
+        if n_views is None:
+            n_views = a.size - view0
+        check(lib().lct_project_ellipsoids(self.h, C.byref(g), _fptr(a), _fptr(c), view0,
+                                           n_views, e.ctypes.data_as(C.POINTER(C.c_double)),
+                                           e.shape[0], _dev(out)))
+
+    def poisson_noise(self, proj, i0: float, seed: int) -> None:
+        check(lib().lct_poisson_noise(self.h, _dev(proj), proj.numel(), C.c_float(i0),
+                                      C.c_uint64(seed)))
+
+
+def scan_geometry(w) -> _lib.CTGeom:
+    """CTGeom fields the projector needs for a workload (ws, wt, ds, dt, dso)."""
+    g = _lib.CTGeom()
+    g.ns, g.nt = w.ns, w.nt
+    g.ws, g.wt = w.central_channel, (w.nt - 1) / 2.0
+    g.ds, g.dt, g.dso = w.ds, w.dt, w.dso
+    return g
+
+
+def synth_scan(ctx: Context, w, noise: bool = True):
+    """Projection file of workload `w` on the device: [data_offset + readings][nt][ns]."""
+    import torch
+    angles, couch = w.angles_couch()
+    g = scan_geometry(w)
+    fan = torch.zeros((w.data_offset + w.readings, w.nt, w.ns), dtype=torch.float32,
+                      device=f"cuda:{ctx.device}")
+    body = fan[w.data_offset:]
+    ctx.project_ellipsoids(g, angles, couch, w.ellipsoids(), body)
+    if noise:
+        ctx.poisson_noise(body, w.i0, w.noise_seed)
+    ctx.synchronize()
+    return fan, angles, couch

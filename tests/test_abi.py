@@ -1,0 +1,72 @@
+"""The C ABI library: it loads, exports every function include/lct.h declares,
+and its host-only functions (planner, block windows, taps) run without a GPU.
+No kernel is launched here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from listctrecons_amd import _lib, recon, scan
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lct_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = declared("lct.h")
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the ctypes table covers all of them
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_version_and_errors():
+    lib = _lib.lib()
+    assert lib.lct_version().decode().startswith("lct ")
+    rc = lib.lct_ramp_taps(0, 0, C.c_float(1.0), None)
+    assert rc == _lib.LCT_ERR_INVALID
+    assert b"lct_ramp_taps" in lib.lct_last_error()
+    with pytest.raises(_lib.LctError):
+        recon.ramp_taps(7, 16, 1.0)
+
+
+def test_host_planner_without_gpu():
+    w = scan.workload("helical_64x736_512x256")
+    a, c = w.angles_couch()
+    pl = recon.plan_helical(w.prm(), a, c)
+    assert (pl.ctg.add_projections, pl.ctg.n_angle_add, pl.ctg.n_angle) == (81, 2148, 1986)
+    assert pl.n_blocks == 8
+    s0 = recon.block_info(pl, c, 0)[0]
+    s1 = recon.block_info(pl, c, 1)[0]
+    # consecutive blocks advance by 32 slices of table travel
+    assert (s1 - s0) == pytest.approx(32 * w.dz / (w.pitch_value / w.n_proj_turn), abs=1.0)
+
+
+def test_planner_range_error_code():
+    w = scan.workload("tiny", readings=300)
+    a, c = w.angles_couch()
+    with pytest.raises(_lib.LctError) as e:
+        recon.plan_helical(w.prm(), a, c)
+    assert e.value.code == _lib.LCT_ERR_RANGE
+    assert "outside of allowed data range" in str(e.value)
+
+
+def test_ctx_create_fails_cleanly_without_gpu():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except Exception:
+        pass
+    h = C.c_void_p()
+    rc = _lib.lib().lct_ctx_create(0, C.byref(h))
+    assert rc < 0 and not h.value
