@@ -42,7 +42,11 @@ enum { LCT_WEIGHT_NONE = 0, LCT_WEIGHT_PI_ORI = 1 };
 enum { LCT_WINDOW_NONE = 0, LCT_WINDOW_HANN = 1, LCT_WINDOW_SHEPP_LOGAN = 2 };
 enum { LCT_FILTER_DIRECT_F32 = 0, LCT_FILTER_TOEPLITZ_BF16X3 = 1 };
 enum { LCT_BP_HELICAL = 0, LCT_BP_PI_ORI = 1 };
-enum { LCT_INTERP_LDS = 0, LCT_INTERP_SAMPLER = 1 };
+/* Bilinear taps of the backprojector: LDS = view windows staged in LDS and
+ * interpolated explicitly; GATHER = taps loaded from HBM through the vector
+ * L1/L2 caches.  (gfx950 exposes no texture sampler to HIP: the image API is
+ * compiled out for this target.) */
+enum { LCT_INTERP_LDS = 0, LCT_INTERP_GATHER = 1 };
 enum { LCT_PIPE_BLOCKS = 0, LCT_PIPE_RESIDENT = 1 };
 
 /* ------------------------------------------------------------ geometry */
@@ -145,6 +149,17 @@ int lct_weighting(lct_ctx *ctx, int kind, const lct_ct_geom *g, float *d_par);
  * over rows = n_angle*nt rows of ns channels, in place, FP32. */
 int lct_filter(lct_ctx *ctx, const lct_ct_geom *g, const float *h_taps, float *d_rows,
                long rows);
+
+/* The ramp convolution of Filter::filter (CUDA/Filter.cu:10-50) written in
+ * the resident backprojector's layout: d_rows [views][nt][ns] f32 ->
+ * d_q [views][ns + 3][nt + 8] of `storage` type, channel-major, channel c at
+ * slot c + 1 and row r at slot r + 4, every other slot 0 (the caller clears
+ * d_q).  algo LCT_FILTER_DIRECT_F32 (FP32 FMA, k ascending) or
+ * LCT_FILTER_TOEPLITZ_BF16X3 (Toeplitz GEMM on BF16 MFMA, hi/lo split
+ * operands, FP32 accumulation).  Needs nt % 8 == 0 (and ns % 4 == 0 for the
+ * Toeplitz algorithm). */
+int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int ns, int nt,
+                            const float *h_taps, const float *d_rows, long views, void *d_q);
 
 /* Backprojector::backproject, CUDA/Backprojector.cu:101-169 (HELICAL) and
  * :252-307 (PI_ORI): d_q [n_angle][nt][ns] f32, h_betas the block's tube

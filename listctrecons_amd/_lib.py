@@ -24,7 +24,7 @@ WEIGHT_NONE, WEIGHT_PI_ORI = 0, 1
 WINDOW_NONE, WINDOW_HANN, WINDOW_SHEPP_LOGAN = 0, 1, 2
 FILTER_DIRECT_F32, FILTER_TOEPLITZ_BF16X3 = 0, 1
 BP_HELICAL, BP_PI_ORI = 0, 1
-INTERP_LDS, INTERP_SAMPLER = 0, 1
+INTERP_LDS, INTERP_GATHER = 0, 1
 PIPE_BLOCKS, PIPE_RESIDENT = 0, 1
 
 
@@ -127,6 +127,8 @@ SIGNATURES = {
     "lct_weighting": (C.c_int, [_vp, C.c_int, _P(CTGeom), _vp]),
     "lct_filter": (C.c_int, [_vp, _P(CTGeom), _fp, _vp, C.c_long]),
     "lct_backproject": (C.c_int, [_vp, C.c_int, _P(CTGeom), _P(ImgGeom), _vp, _fp, _vp]),
+    "lct_filter_to_bp_layout": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _vp,
+                                          C.c_long, _vp]),
     "lct_reconstruct": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long, _vp]),
     "lct_get_timings": (C.c_int, [_vp, _P(Timings)]),
     "lct_set_profiling": (C.c_int, [_vp, C.c_int]),

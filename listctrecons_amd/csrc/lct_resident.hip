@@ -319,6 +319,99 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     }
 }
 
+// The same pair-folded backprojection with every bilinear tap gathered from
+// HBM through the vector L1 / L2 caches (no LDS staging, no barriers).  The
+// zero guard slots of Q give the reference's border behaviour.
+template <typename T>
+__global__ __launch_bounds__(256) void bp_fold_gather_kernel(FoldBpArgs A)
+{
+    int wg = blockIdx.x;
+    if (A.xcd_remap) wg = (wg & 7) * (A.n_wg >> 3) + (wg >> 3);
+    const int tiles = A.n_tiles_x * A.n_tiles_y;
+    const int blk = wg / tiles;
+    const int t = wg - blk * tiles;
+    const int tx0 = (t % A.n_tiles_x) * kTile, ty0 = (t / A.n_tiles_x) * kTile;
+    const T *Q = static_cast<const T *>(A.q);
+    const int rp = A.rp;
+    const int ix = tx0 + int(threadIdx.x & (kTile - 1));
+    const int iy = ty0 + int(threadIdx.x / kTile);
+    const int *st = A.slab_tab + blk * 34;
+    const int qoff = st[0], slice0 = st[1];
+    int am[32];
+#pragma unroll
+    for (int z = 0; z < 32; z++) am[z] = st[2 + z];
+    const float xc = (float(ix) - A.wx) * A.dx;
+    const float yc = (float(iy) - A.wy) * A.dy;
+    float acc[32];
+#pragma unroll
+    for (int z = 0; z < 32; z++) acc[z] = 0.0f;
+    const int jlo = am[0] - A.h, jhi = am[31];
+    for (int j = jlo; j < jhi; j++) {
+        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
+        const float c2 = A.cosb[qoff + j + A.h], s2v = A.sinb[qoff + j + A.h];
+        const float phat = -xc * c - yc * s;
+        const float theta = asinf(phat * A.inv_dso);
+        const float delta = theta * A.dso_ds;
+        const float p1 = fminf(fmaxf(A.ws + delta, -1.0f), A.ns_f);
+        const float p2 = fminf(fmaxf(A.ws - delta, -1.0f), A.ns_f);
+        const float fp1 = floorf(p1), fp2 = floorf(p2);
+        const float ap1 = p1 - fp1, ap2 = p2 - fp2;
+        // channel slot = channel + 1 (slots 0 and ns+1, ns+2 are zero)
+        const T *b1 = Q + size_t(qoff + j) * A.view_stride + size_t(int(fp1) + 1) * rp;
+        const T *b2 = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(int(fp2) + 1) * rp;
+        const float root = sqrtf(A.dso2 - phat * phat);
+        const float lhat = root + xc * s - yc * c;
+        const float lhat2 = root + xc * s2v - yc * c2;
+        const float tz = theta * A.inv_orbit;
+        const float zr = A.z0 + (float(j) + tz) * A.dzv;
+        const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
+        const float k1 = A.kc / lhat, k2 = A.kc / lhat2;
+        const float o1 = -zr * k1, o2 = -zr2 * k2;
+#pragma unroll
+        for (int z = 0; z < 32; z++) {
+            if (am[z] > j && am[z] <= j + A.h) {
+                const float zc = (float(z) - A.wz) * A.dz;
+                const float qh = fmaf(zc, k1, o1);
+                const float qh2 = fmaf(zc, k2, o2);
+                const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
+                const float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
+                const float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
+                const float w2 = out1 ? (out2 ? 0.5f : 1.0f) : (out2 ? 0.0f : 1.0f - wb);
+                const float r1 = fminf(fmaxf(qh + A.wt, -1.0f), A.nt_f);
+                const float r2 = fminf(fmaxf(qh2 + A.wt, -1.0f), A.nt_f);
+                const float fr1 = floorf(r1), fr2 = floorf(r2);
+                const float ar1 = r1 - fr1, ar2 = r2 - fr2;
+                const int i1 = int(fr1) + kRowPad, i2 = int(fr2) + kRowPad;
+                const float a00 = Cvt<T>::from(b1[i1]), a01 = Cvt<T>::from(b1[i1 + 1]);
+                const float a10 = Cvt<T>::from(b1[rp + i1]), a11 = Cvt<T>::from(b1[rp + i1 + 1]);
+                const float c00 = Cvt<T>::from(b2[i2]), c01 = Cvt<T>::from(b2[i2 + 1]);
+                const float c10 = Cvt<T>::from(b2[rp + i2]), c11 = Cvt<T>::from(b2[rp + i2 + 1]);
+                const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
+                const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
+                const float t1 = fmaf(ar1, ha - la, la);
+                const float t2 = fmaf(ar2, hc - lc, lc);
+                acc[z] = fmaf(w1, t1, fmaf(w2, t2, acc[z]));
+            }
+        }
+    }
+    if (ix < A.nx && iy < A.ny) {
+        float *o = A.vol + (size_t(slice0) * A.ny + iy) * A.nx + ix;
+        const size_t plane = size_t(A.nx) * A.ny;
+#pragma unroll
+        for (int z = 0; z < 32; z++) o[size_t(z) * plane] = acc[z] * A.scale;
+    }
+}
+
+template <typename T>
+static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
+{
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
+    hipLaunchKernelGGL(bp_fold_gather_kernel<T>, dim3(unsigned(a.n_wg)), dim3(256), 0, ctx->stream, a);
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
 template <typename T>
 static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
 {
@@ -411,8 +504,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     // Can the pair-folded backprojector be used?
     int amin[32], amax[32], amid[32];
     bp_windows(g, ig, amin, amax, amid);
-    bool fold = (g.n_proj_turn % 2 == 0) && o.bp_kind == LCT_BP_HELICAL && (g.nt % 8 == 0) &&
-                o.interp == LCT_INTERP_LDS;
+    bool fold = (g.n_proj_turn % 2 == 0) && o.bp_kind == LCT_BP_HELICAL && (g.nt % 8 == 0);
     for (int z = 0; z < 32 && fold; z++)
         fold = (amid[z] - h >= 0) && (amid[z] + h < g.n_angle) && (z == 0 || amid[z] >= amid[z - 1]);
     for (long v = R.g0; v + h < R.g0 + R.nviews && fold; v++) {
@@ -499,7 +591,10 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_bytes, ctx->stream));
     LCT_CHECK(tic());
     const long rows = long(R.nviews) * g.nt;
-    if (o.storage == LCT_F32)
+    if (o.filter_algo == LCT_FILTER_TOEPLITZ_BF16X3 && g.ns % 4 == 0)
+        LCT_CHECK(launch_filter_toeplitz(ctx, o.storage, g.ns, g.nt, taps.data(), P, d_q, rows,
+                                         cp, rp));
+    else if (o.storage == LCT_F32)
         LCT_CHECK(launch_filter_to_q<float>(ctx, g.ns, g.nt, static_cast<const float *>(d_taps),
                                             P, static_cast<float *>(d_q), rows, cp, rp));
     else if (o.storage == LCT_F16)
@@ -561,7 +656,14 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.nt_f = float(g.nt);
     a.scale = float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
     LCT_CHECK(tic());
-    if (o.storage == LCT_F32)
+    if (o.interp == LCT_INTERP_GATHER) {
+        if (o.storage == LCT_F32)
+            LCT_CHECK(launch_bp_fold_gather<float>(ctx, a, plan.n_blocks));
+        else if (o.storage == LCT_F16)
+            LCT_CHECK(launch_bp_fold_gather<__half>(ctx, a, plan.n_blocks));
+        else
+            LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, a, plan.n_blocks));
+    } else if (o.storage == LCT_F32)
         LCT_CHECK(launch_bp_fold<float>(ctx, a, plan.n_blocks));
     else if (o.storage == LCT_F16)
         LCT_CHECK(launch_bp_fold<__half>(ctx, a, plan.n_blocks));
@@ -573,3 +675,31 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
 }
 
 }  // namespace lct
+
+using namespace lct;
+
+extern "C" int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int ns, int nt,
+                                       const float *h_taps, const float *d_rows, long views,
+                                       void *d_q)
+{
+    if (!ctx || !h_taps || !d_rows || !d_q || ns <= 0 || nt <= 0 || nt % 8 != 0 || views < 0 ||
+        storage < 0 || storage > 2 || algo < 0 || algo > 1)
+        return set_error(LCT_ERR_INVALID, "lct_filter_to_bp_layout: bad argument");
+    if (algo == LCT_FILTER_TOEPLITZ_BF16X3 && ns % 4 != 0)
+        return set_error(LCT_ERR_UNSUPPORTED, "lct_filter_to_bp_layout: Toeplitz needs ns % 4 == 0");
+    LCT_CHECK_HIP(hipSetDevice(ctx->device));
+    const int cp = ns + 3, rp = nt + 8;
+    const long rows = views * nt;
+    if (algo == LCT_FILTER_TOEPLITZ_BF16X3)
+        return launch_filter_toeplitz(ctx, storage, ns, nt, h_taps, d_rows, d_q, rows, cp, rp);
+    void *d_taps = nullptr;
+    LCT_CHECK(scratch(ctx, kScratchTaps, size_t(2) * ns * sizeof(float), &d_taps));
+    LCT_CHECK(upload(ctx, d_taps, h_taps, size_t(2) * ns * sizeof(float)));
+    const float *t = static_cast<const float *>(d_taps);
+    if (storage == LCT_F32)
+        return launch_filter_to_q<float>(ctx, ns, nt, t, d_rows, static_cast<float *>(d_q), rows, cp, rp);
+    if (storage == LCT_F16)
+        return launch_filter_to_q<__half>(ctx, ns, nt, t, d_rows, static_cast<__half *>(d_q), rows, cp, rp);
+    return launch_filter_to_q<__hip_bfloat16>(ctx, ns, nt, t, d_rows,
+                                              static_cast<__hip_bfloat16 *>(d_q), rows, cp, rp);
+}

@@ -1,0 +1,283 @@
+// lct_toeplitz.hip -- the ramp convolution (CUDA/Filter.cu:42-46) as a
+// Toeplitz GEMM on gfx950 BF16 matrix cores with split operands.
+//
+//   Q[(v,r)][ch] = sum_k T[ch][k] P[(v,r)][k],   T[ch][k] = h[ch - k + ns]
+//
+// C (M = channels) = A (M x K, Toeplitz taps) * B (K x N, N = every row of
+// every view).  Each FP32 operand x is split into bf16 hi = rn(x) and
+// lo = rn(x - hi); hi*hi + hi*lo + lo*hi (three v_mfma_f32_16x16x32_bf16 per
+// fragment, FP32 accumulation) keeps the products to ~2^-16 relative,
+// which is FP32-level for the filtered projections.  The result is written in
+// the backprojector's channel-major layout Q[view][slot_ch][slot_row]
+// (see lct_resident.hip) as f32, f16 or bf16.
+//
+// A operand: a lane needs 8 consecutive taps h[i - k + ns] for k = k0..k0+7,
+// i.e. 8 consecutive entries of the reversed taps starting at an offset that
+// is not 16-byte aligned in general; the workgroup keeps 8 copies of the
+// reversed taps shifted by 0..7 elements in LDS so every fragment is one
+// aligned 16-byte read.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lct_internal.hpp"
+
+namespace lct {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kTM = 128;     // channels per workgroup tile (8 fragments of 16)
+constexpr int kTN = 64;      // (view,row) columns per tile (4 fragments of 16)
+constexpr int kTK = 32;      // K per MFMA step
+constexpr int kBRow = 40;    // LDS row of one B column: 32 k + 8 pad (bf16)
+constexpr int kRowPadT = 4;  // zero row slots before row 0 in Q (as lct_resident.hip)
+
+struct ToeplitzArgs {
+    const unsigned short *taps_hi;  // [8][L] shifted reversed taps (bf16 bits)
+    const unsigned short *taps_lo;
+    const float *in;                // P [(v,r)][ns] f32
+    void *q;                        // Q [v][cp][rp]
+    long n_cols;                    // V * nt
+    int ns, nt, cp, rp, kp, L, P, n_m_tiles, n_n_wg;
+};
+
+__device__ __forceinline__ unsigned short bf16_rn(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f(unsigned short b) { return __uint_as_float(unsigned(b) << 16); }
+
+template <typename T> __device__ __forceinline__ T to_store(float x);
+template <> __device__ __forceinline__ float to_store<float>(float x) { return x; }
+template <> __device__ __forceinline__ __half to_store<__half>(float x) { return __float2half_rn(x); }
+template <> __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(float x)
+{
+    return __float2bfloat16(x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
+    unsigned short *th = smem;                 // [8][L]
+    unsigned short *tl = th + 8 * A.L;         // [8][L]
+    unsigned short *bh = tl + 8 * A.L;         // [2][kTN][kBRow]
+    unsigned short *bl = bh + 2 * kTN * kBRow; // [2][kTN][kBRow]
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int mt = blockIdx.x % A.n_m_tiles;
+    const int nw = blockIdx.x / A.n_m_tiles;
+    const int m0 = mt * kTM;
+
+    // taps: 16-byte copies from global (L2 resident)
+    {
+        const uint4 *sh = reinterpret_cast<const uint4 *>(A.taps_hi);
+        const uint4 *sl = reinterpret_cast<const uint4 *>(A.taps_lo);
+        uint4 *dh = reinterpret_cast<uint4 *>(th);
+        uint4 *dl = reinterpret_cast<uint4 *>(tl);
+        const int n16 = 8 * A.L / 8;
+        for (int i = tid; i < n16; i += 256) {
+            dh[i] = sh[i];
+            dl[i] = sl[i];
+        }
+    }
+
+    const int li = lane & 15;       // fragment row / column within 16
+    const int lk = (lane >> 4) * 8; // k offset of this lane's 8 values
+    const int n_ksteps = A.kp / kTK;
+
+    // B staging: thread -> column tid / 4, k group (tid % 4) * 8
+    const int bcol = tid >> 2, bk = (tid & 3) * 8;
+    auto stage_b = [&](long n0, int ks, int buf, float (&x)[8]) {
+        const long col = n0 + bcol;
+        const int k = ks * kTK + bk;
+        if (col < A.n_cols && k < A.ns) {
+            const float *src = A.in + size_t(col) * A.ns + k;
+            if (k + 8 <= A.ns) {
+                const float4 a = *reinterpret_cast<const float4 *>(src);
+                const float4 b = *reinterpret_cast<const float4 *>(src + 4);
+                x[0] = a.x; x[1] = a.y; x[2] = a.z; x[3] = a.w;
+                x[4] = b.x; x[5] = b.y; x[6] = b.z; x[7] = b.w;
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; j++) x[j] = (k + j < A.ns) ? src[j] : 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; j++) x[j] = 0.0f;
+        }
+        (void)buf;
+    };
+    auto commit_b = [&](int buf, const float (&x)[8]) {
+        unsigned short hi[8], lo[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            hi[j] = bf16_rn(x[j]);
+            lo[j] = bf16_rn(x[j] - bf16_to_f(hi[j]));
+        }
+        uint4 vh, vl;
+        vh.x = hi[0] | (unsigned(hi[1]) << 16); vh.y = hi[2] | (unsigned(hi[3]) << 16);
+        vh.z = hi[4] | (unsigned(hi[5]) << 16); vh.w = hi[6] | (unsigned(hi[7]) << 16);
+        vl.x = lo[0] | (unsigned(lo[1]) << 16); vl.y = lo[2] | (unsigned(lo[3]) << 16);
+        vl.z = lo[4] | (unsigned(lo[5]) << 16); vl.w = lo[6] | (unsigned(lo[7]) << 16);
+        *reinterpret_cast<uint4 *>(bh + (buf * kTN + bcol) * kBRow + bk) = vh;
+        *reinterpret_cast<uint4 *>(bl + (buf * kTN + bcol) * kBRow + bk) = vl;
+    };
+
+    __syncthreads();  // taps in LDS
+
+    const long n_tiles = (A.n_cols + kTN - 1) / kTN;
+    for (long nt_i = nw; nt_i < n_tiles; nt_i += A.n_n_wg) {
+        const long n0 = nt_i * kTN;
+        f32x4 acc[2][4];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int f = 0; f < 4; f++) acc[a][f] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+        float xb[8];
+        stage_b(n0, 0, 0, xb);
+        commit_b(0, xb);
+        __syncthreads();
+        for (int ks = 0; ks < n_ksteps; ks++) {
+            const int buf = ks & 1;
+            if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, buf ^ 1, xb);
+            // A fragments of this wave's two 16-channel slices
+            bf16x8 ah[2], al[2];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int i = m0 + (2 * wave + a) * 16 + li;   // output channel
+                const int o = A.ns - 1 - i + ks * kTK + lk + A.P;  // >= 0 by the choice of P
+                const int s = o & 7, u = o - s;
+                ah[a] = *reinterpret_cast<const bf16x8 *>(th + s * A.L + u);
+                al[a] = *reinterpret_cast<const bf16x8 *>(tl + s * A.L + u);
+            }
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                const int c = f * 16 + li;
+                const bf16x8 b_hi = *reinterpret_cast<const bf16x8 *>(bh + (buf * kTN + c) * kBRow + lk);
+                const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(bl + (buf * kTN + c) * kBRow + lk);
+#pragma unroll
+                for (int a = 0; a < 2; a++) {
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], b_hi, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], b_lo, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], b_hi, acc[a][f], 0, 0, 0);
+                }
+            }
+            if (ks + 1 < n_ksteps) commit_b(buf ^ 1, xb);
+            __syncthreads();
+        }
+        // epilogue: D[i][n] -> Q[v][ch + 1][r + 4]
+        T *q = static_cast<T *>(A.q);
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            const long col = n0 + f * 16 + li;
+            if (col >= A.n_cols) continue;
+            const long v = col / A.nt;
+            const int r = int(col - v * A.nt);
+            T *base = q + size_t(v) * A.cp * A.rp + r + kRowPadT;
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int ch = m0 + (2 * wave + a) * 16 + (lane >> 4) * 4 + j;
+                    if (ch < A.ns) base[size_t(ch + 1) * A.rp] = to_store<T>(acc[a][f][j]);
+                }
+            }
+        }
+    }
+}
+
+static unsigned short host_bf16_rn(float x)
+{
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+
+static float host_bf16_f(unsigned short b)
+{
+    unsigned u = unsigned(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+template <typename T>
+int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, const float *d_in,
+                             T *d_q, long n_cols, int cp, int rp)
+{
+    const int mp = (ns + kTM - 1) / kTM * kTM;
+    const int kp = (ns + kTK - 1) / kTK * kTK;
+    const int P = ((std::max(mp - ns, 0) + 7) / 8) * 8;
+    const int L = ((ns + kp + P + 16 + 7) / 8) * 8;
+    // reversed taps hr[t] = h[2 ns - 1 - t], padded by P, in 8 shifted copies
+    std::vector<float> hr(size_t(L) + 8, 0.0f);
+    for (int t = 0; t < 2 * ns; t++) hr[size_t(t) + P] = h_taps[2 * ns - 1 - t];
+    std::vector<unsigned short> hi(8 * size_t(L)), lo(8 * size_t(L));
+    for (int s = 0; s < 8; s++)
+        for (int u = 0; u < L; u++) {
+            const float x = hr[size_t(u) + s];
+            const unsigned short b = host_bf16_rn(x);
+            hi[size_t(s) * L + u] = b;
+            lo[size_t(s) * L + u] = host_bf16_rn(x - host_bf16_f(b));
+        }
+    void *dt = nullptr;
+    LCT_CHECK(scratch(ctx, kScratchTaps, hi.size() * 4, &dt));
+    unsigned short *d_hi = static_cast<unsigned short *>(dt);
+    unsigned short *d_lo = d_hi + hi.size();
+    LCT_CHECK(upload(ctx, d_hi, hi.data(), hi.size() * 2));
+    LCT_CHECK(upload(ctx, d_lo, lo.data(), lo.size() * 2));
+
+    ToeplitzArgs a;
+    a.taps_hi = d_hi;
+    a.taps_lo = d_lo;
+    a.in = d_in;
+    a.q = d_q;
+    a.n_cols = n_cols;
+    a.ns = ns;
+    a.nt = nt;
+    a.cp = cp;
+    a.rp = rp;
+    a.kp = kp;
+    a.L = L;
+    a.P = P;
+    a.n_m_tiles = mp / kTM;
+    const long n_tiles = (n_cols + kTN - 1) / kTN;
+    a.n_n_wg = int(std::min<long>(n_tiles, std::max<long>(1, 2048 / a.n_m_tiles)));
+    const size_t shm = (size_t(16) * L + size_t(4) * kTN * kBRow) * sizeof(unsigned short);
+    if (shm > 160 * 1024) return set_error(LCT_ERR_UNSUPPORTED, "toeplitz filter: ns too large");
+    if (shm > 64 * 1024)
+        LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&toeplitz_filter_kernel<T>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
+    hipLaunchKernelGGL(toeplitz_filter_kernel<T>, dim3(unsigned(a.n_m_tiles * a.n_n_wg)), dim3(256),
+                       shm, ctx->stream, a);
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
+int launch_filter_toeplitz(lct_ctx *ctx, int storage, int ns, int nt, const float *h_taps,
+                           const float *d_in, void *d_q, long n_cols, int cp, int rp)
+{
+    if (ns % 4 != 0)
+        return set_error(LCT_ERR_UNSUPPORTED, "toeplitz filter: ns must be a multiple of 4");
+    if (storage == LCT_F32)
+        return launch_filter_toeplitz_t<float>(ctx, ns, nt, h_taps, d_in, static_cast<float *>(d_q),
+                                               n_cols, cp, rp);
+    if (storage == LCT_F16)
+        return launch_filter_toeplitz_t<__half>(ctx, ns, nt, h_taps, d_in, static_cast<__half *>(d_q),
+                                                n_cols, cp, rp);
+    return launch_filter_toeplitz_t<__hip_bfloat16>(ctx, ns, nt, h_taps, d_in,
+                                                    static_cast<__hip_bfloat16 *>(d_q), n_cols, cp, rp);
+}
+
+}  // namespace lct

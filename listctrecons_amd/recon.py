@@ -106,6 +106,13 @@ def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
     if name == "resident_direct_f16":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16)
+    if name == "fast_f32_toeplitz":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window,
+                       filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
+    if name.endswith("_gather"):
+        o = preset(name[: -len("_gather")], window)
+        o.interp = _lib.INTERP_GATHER
+        return o
     raise KeyError(name)
 
 
@@ -158,6 +165,13 @@ class Context:
     def filter(self, g: _lib.CTGeom, taps: np.ndarray, rows, n_rows: int) -> None:
         t = np.ascontiguousarray(taps, np.float32)
         check(lib().lct_filter(self.h, C.byref(g), _fptr(t), _dev(rows), n_rows))
+
+    def filter_to_bp_layout(self, algo: int, storage: int, ns: int, nt: int, taps: np.ndarray,
+                            rows, views: int, q) -> None:
+        """Ramp filter of rows [views][nt][ns] into Q [views][ns+3][nt+8] (q pre-zeroed)."""
+        t = np.ascontiguousarray(taps, np.float32)
+        check(lib().lct_filter_to_bp_layout(self.h, algo, storage, ns, nt, _fptr(t), _dev(rows),
+                                            views, _dev(q)))
 
     def backproject(self, kind: int, g: _lib.CTGeom, ig: _lib.ImgGeom, q, betas, img) -> None:
         b = np.ascontiguousarray(betas, np.float32)

@@ -146,6 +146,32 @@ class Workload:
         return np.concatenate(parts, axis=0)
 
 
+def write_scan_files(w: Workload, directory: str, fan: np.ndarray | None = None,
+                     recon_type: str | None = None) -> str:
+    """Writes the reference's input files for workload `w` into `directory`
+    (trailing '/' added: the reference joins paths by concatenation):
+      angles.raw / couch.raw   float32, ViewOffset leading values then Readings
+      proj.raw                 float32 [DataOffset + Readings][nt][ns] (zeros if fan is None)
+      scan.prm                 all 23 keys (plus ReconType when given)
+    Returns the .prm path."""
+    import os
+    d = directory.rstrip("/") + "/"
+    os.makedirs(d, exist_ok=True)
+    a, c = w.angles_couch()
+    pad = np.zeros(w.view_offset, np.float32)
+    np.concatenate([pad + 9.0, a]).astype(np.float32).tofile(d + "angles.raw")
+    np.concatenate([pad - 9.0, c]).astype(np.float32).tofile(d + "couch.raw")
+    if fan is None:
+        fan = np.zeros((w.data_offset + w.readings, w.nt, w.ns), np.float32)
+    np.ascontiguousarray(fan, np.float32).tofile(d + "proj.raw")
+    text = w.prm_text(d, d)
+    if recon_type:
+        text += f"ReconType: {recon_type}\n"
+    with open(d + "scan.prm", "w") as f:
+        f.write(text)
+    return d + "scan.prm"
+
+
 def _rot(phi: float, theta: float, psi: float) -> np.ndarray:
     """Body->world rotation Rz(phi) Rx(theta) Rz(psi) (radians)."""
     def rz(a):

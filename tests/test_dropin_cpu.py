@@ -1,0 +1,92 @@
+"""The C++ drop-in layer without a GPU: the .prm parser against the
+reference's own Src/FileParams.cpp (built into oracle/_ref by oracle/ref.mk
+when the reference sources are present), the CLI's planner against the
+oracle, and the reference's exit(1) paths."""
+import os
+import subprocess
+
+import pytest
+
+from listctrecons_amd import scan
+from oracle import oracle as O
+from helpers import oracle_plan
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "listctrecons_amd", "bin", "LISTCTRecons")
+REF_PARSER = os.path.join(ROOT, "oracle", "_ref", "ref_fileparams")
+
+
+def run(*args):
+    return subprocess.run(list(args), stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+@pytest.fixture(scope="module")
+def files(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("scan"))
+    w = scan.workload("tiny", view_offset=3, data_offset=5)
+    return w, scan.write_scan_files(w, d)
+
+
+def prm_variants(path, tmp):
+    base = open(path).read()
+    out = {"plain": base}
+    out["tablefeed"] = base.replace("PitchValue:", "TableFeed:")
+    out["extra_tokens"] = "Comment: hello\n" + base + "SmoothWindow: hann\nReconType: x\n"
+    out["one_line"] = " ".join(base.split())
+    paths = {}
+    for k, v in out.items():
+        p = os.path.join(tmp, k + ".prm")
+        with open(p, "w") as f:
+            f.write(v)
+        paths[k] = p
+    return paths
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PARSER), reason="reference parser not built")
+def test_prm_parser_matches_reference(files, tmp_path):
+    w, prm = files
+    for name, p in prm_variants(prm, str(tmp_path)).items():
+        ref = run(REF_PARSER, p)
+        ours = run(CLI, "--prm", p)
+        assert ref.returncode == 0 and ours.returncode == 0, name
+        assert ours.stdout == ref.stdout, name
+
+
+@pytest.mark.skipif(not os.path.exists(REF_PARSER), reason="reference parser not built")
+def test_missing_key_exits_like_reference(files, tmp_path):
+    w, prm = files
+    text = "".join(l for l in open(prm) if not l.startswith("Dz:"))
+    p = str(tmp_path / "missing.prm")
+    open(p, "w").write(text)
+    ref = run(REF_PARSER, p)
+    ours = run(CLI, "--prm", p)
+    assert ref.returncode == ours.returncode == 1
+    assert "Parameter not found in prm file:Dz" in ref.stdout
+    assert ours.stdout == ref.stdout
+
+
+def test_cli_plan_matches_oracle(files):
+    w, prm = files
+    out = run(CLI, "--plan", prm)
+    assert out.returncode == 0, out.stdout
+    pl, a, c = oracle_plan(w)
+    line = [l for l in out.stdout.splitlines() if l.startswith("add ")][0].split()
+    vals = dict(zip(line[0::2], line[1::2]))
+    assert int(vals["add"]) == pl.ctg.add_projections
+    assert int(vals["n_angle_add"]) == pl.ctg.n_angle_add
+    assert int(vals["n_angle"]) == pl.ctg.n_angle
+    assert int(vals["n_slices_recon"]) == pl.n_slices_recon
+    blocks = [l.split() for l in out.stdout.splitlines() if l.startswith("block ")]
+    for b in blocks:
+        s, e, _, _ = O.block_info(pl, c, int(b[1]))
+        assert (int(b[3]), int(b[5])) == (s, e)
+
+
+def test_cli_errors(tmp_path):
+    assert run(CLI).returncode == 1
+    r = run(CLI, "--plan", str(tmp_path / "nope.prm"))
+    assert r.returncode == 1 and "Parameter file not found" in r.stdout
+    w = scan.workload("tiny", readings=300)
+    prm = scan.write_scan_files(w, str(tmp_path / "short"))
+    r = run(CLI, "--plan", prm)
+    assert r.returncode == 1 and "outside of allowed data range" in r.stdout

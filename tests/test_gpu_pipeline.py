@@ -40,7 +40,8 @@ def cfg1():
     return w, fan, ref
 
 
-@pytest.mark.parametrize("preset,tol", [("plain", 1e-4), ("fast_f32", 1e-4)])
+@pytest.mark.parametrize("preset,tol", [("plain", 1e-4), ("fast_f32", 1e-4),
+                                        ("fast_f32_gather", 1e-4), ("fast_f32_toeplitz", 1e-4)])
 def test_fp32_paths_match_oracle(ctx, cfg1, preset, tol):
     w, fan, ref = cfg1
     print(f"{preset} on {w.name}", flush=True)
@@ -51,7 +52,8 @@ def test_fp32_paths_match_oracle(ctx, cfg1, preset, tol):
     assert err <= tol
 
 
-@pytest.mark.parametrize("preset", ["resident_direct_f16", "mixed_f16", "mixed_bf16"])
+@pytest.mark.parametrize("preset", ["resident_direct_f16", "mixed_f16", "mixed_bf16",
+                                    "mixed_f16_gather"])
 def test_reduced_storage_within_1hu(ctx, cfg1, preset):
     w, fan, ref = cfg1
     print(f"{preset} on {w.name}", flush=True)

@@ -82,6 +82,33 @@ def test_filter(ctx, data, name, window):
 
 
 @pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("algo,storage,tol", [(0, 0, 2e-6), (1, 0, 2e-5), (1, 1, 1e-3),
+                                              (1, 2, 5e-3), (0, 1, 1e-3)])
+def test_filter_into_backprojector_layout(ctx, data, name, algo, storage, tol):
+    """Direct and BF16x3 Toeplitz-MFMA filters, Q[view][ns+3][nt+8] output."""
+    w, pl, a, c, fan, win, s = data[name]
+    g = pl.ctg
+    print(f"filter->Q {name} algo={algo} storage={storage}", flush=True)
+    rows = O.rebin(g, win, kind=1).astype(np.float32)
+    taps = recon.ramp_taps(2, g.ns, g.ds)
+    ref = O.filter_rows(taps.astype(np.float64), rows.astype(np.float64))
+    dt = {0: torch.float32, 1: torch.float16, 2: torch.bfloat16}[storage]
+    q = torch.zeros((g.n_angle, g.ns + 3, g.nt + 8), dtype=dt, device="cuda")
+    ctx.filter_to_bp_layout(algo, storage, g.ns, g.nt, taps, torch.from_numpy(rows).cuda(),
+                            g.n_angle, q)
+    ctx.synchronize()
+    qn = q.float().cpu().numpy()
+    got = qn[:, 1:g.ns + 1, 4:g.nt + 4].transpose(0, 2, 1)
+    err = rel_rmse(got, ref)
+    print(f"  rel rmse {err:.2e}", flush=True)
+    assert err < tol
+    # every guard slot stays zero
+    mask = np.ones(qn.shape, bool)
+    mask[:, 1:g.ns + 1, 4:g.nt + 4] = False
+    assert not qn[mask].any()
+
+
+@pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("kind", [0, 1])
 def test_backproject(ctx, data, name, kind):
     w, pl, a, c, fan, win, s = data[name]
