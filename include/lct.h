@@ -218,6 +218,39 @@ int lct_project_ellipsoids(lct_ctx *ctx, const lct_ct_geom *g, const float *h_an
  * reproducible for a given seed. */
 int lct_poisson_noise(lct_ctx *ctx, float *d_proj, size_t n, float i0, uint64_t seed);
 
+/* Reprojection of a voxel volume d_vol [vol_nz][ny][nx] (voxel centres
+ * x = (ix - wx) dx, y = (iy - wy) dy, z = vol_z0 + iz dz; nz of vg is
+ * ignored) along the rays of lct_project_ellipsoids: trilinear samples at the
+ * midpoints of equal steps of at most `step` mm inside the voxel-centre box.
+ * d_out [n_views][nt][ns].  Not part of the reference: it scores the learned
+ * correction's reprojection residual. */
+int lct_forward_project(lct_ctx *ctx, const lct_ct_geom *g, const float *h_angles,
+                        const float *h_couch, long view0, long n_views, const float *d_vol,
+                        const lct_img_geom *vg, float vol_z0, int vol_nz, float step,
+                        float *d_out);
+
+/* ------------------------------------------- learned artifact correction
+ * Not part of the reference.  A 3D residual CNN, y = x + net(x), net = six
+ * 3x3x3 convolutions 1 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 with ReLU after the
+ * first five, zero 'same' padding.  FP32 master weights packed as
+ * [W1 27x32 | b1 32 | (W 27x32x32 | b 32) x 4 | W6 27x32 | b6 1],
+ * W[tap][co][ci], tap = (dz+1)*9 + (dy+1)*3 + (dx+1); bf16 operand copies
+ * are refreshed after every update.  Volumes are f32 [B][D][H][W]. */
+typedef struct lct_cnn lct_cnn;
+long lct_cnn_param_count(void);
+int lct_cnn_create(lct_ctx *ctx, uint64_t seed, lct_cnn **out);
+int lct_cnn_destroy(lct_cnn *net);
+int lct_cnn_get_params(lct_cnn *net, float *h_params);
+int lct_cnn_set_params(lct_cnn *net, const float *h_params);
+/* Gradients of the last lct_cnn_loss_grad, same packing as the parameters. */
+int lct_cnn_get_grads(lct_cnn *net, float *h_grads);
+int lct_cnn_forward(lct_cnn *net, const float *d_x, int B, int D, int H, int W, float *d_y);
+/* Forward, loss = mean((y - t)^2), and backward into the gradient buffer. */
+int lct_cnn_loss_grad(lct_cnn *net, const float *d_x, const float *d_t, int B, int D, int H,
+                      int W, float *h_loss);
+/* One Adam update of the master weights from the stored gradients. */
+int lct_cnn_adam_step(lct_cnn *net, float lr, float beta1, float beta2, float eps);
+
 #ifdef __cplusplus
 }
 #endif

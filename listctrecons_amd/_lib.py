@@ -135,6 +135,8 @@ SIGNATURES = {
     "lct_project_ellipsoids": (C.c_int, [_vp, _P(CTGeom), _fp, _fp, C.c_long, C.c_long, _dp,
                                          C.c_int, _vp]),
     "lct_poisson_noise": (C.c_int, [_vp, _vp, C.c_size_t, C.c_float, C.c_uint64]),
+    "lct_forward_project": (C.c_int, [_vp, _P(CTGeom), _fp, _fp, C.c_long, C.c_long, _vp,
+                                      _P(ImgGeom), C.c_float, C.c_int, C.c_float, _vp]),
 }
 
 _lib = None

@@ -210,6 +210,18 @@ class Context:
                                            n_views, e.ctypes.data_as(C.POINTER(C.c_double)),
                                            e.shape[0], _dev(out)))
 
+    def forward_project(self, g: _lib.CTGeom, angles, couch, vol, ig: _lib.ImgGeom,
+                        z0: float, out, view0: int = 0, n_views: int | None = None,
+                        step: float = 0.25) -> None:
+        """Reprojection of vol [nz][ny][nx] (slice k at z0 + k dz) into out [views][nt][ns]."""
+        a = np.ascontiguousarray(angles, np.float32)
+        c = np.ascontiguousarray(couch, np.float32)
+        if n_views is None:
+            n_views = a.size - view0
+        check(lib().lct_forward_project(self.h, C.byref(g), _fptr(a), _fptr(c), view0, n_views,
+                                        _dev(vol), C.byref(ig), C.c_float(z0), vol.shape[0],
+                                        C.c_float(step), _dev(out)))
+
     def poisson_noise(self, proj, i0: float, seed: int) -> None:
         check(lib().lct_poisson_noise(self.h, _dev(proj), proj.numel(), C.c_float(i0),
                                       C.c_uint64(seed)))
