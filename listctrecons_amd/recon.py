@@ -249,3 +249,60 @@ def synth_scan(ctx: Context, w, noise: bool = True):
         ctx.poisson_noise(body, w.i0, w.noise_seed)
     ctx.synchronize()
     return fan, angles, couch
+
+
+class Cnn:
+    """The learned residual correction y = x + net(x) (lct_cnn_* in lct.h)."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        h = C.c_void_p()
+        check(lib().lct_cnn_create(ctx.h, C.c_uint64(seed), C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.n_params = int(lib().lct_cnn_param_count())
+
+    def close(self) -> None:
+        if self.h:
+            lib().lct_cnn_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        check(lib().lct_cnn_get_params(self.h, _fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        q = np.ascontiguousarray(p, np.float32)
+        assert q.size == self.n_params
+        check(lib().lct_cnn_set_params(self.h, _fptr(q)))
+
+    def grads(self) -> np.ndarray:
+        g = np.empty(self.n_params, np.float32)
+        check(lib().lct_cnn_get_grads(self.h, _fptr(g)))
+        return g
+
+    def forward(self, x, y=None):
+        """x, y: CUDA float32 tensors [B][D][H][W]."""
+        import torch
+        if y is None:
+            y = torch.empty_like(x)
+        B, D, H, W = x.shape
+        check(lib().lct_cnn_forward(self.h, _dev(x), B, D, H, W, _dev(y)))
+        return y
+
+    def loss_grad(self, x, t) -> float:
+        B, D, H, W = x.shape
+        loss = np.zeros(1, np.float32)
+        check(lib().lct_cnn_loss_grad(self.h, _dev(x), _dev(t), B, D, H, W, _fptr(loss)))
+        return float(loss[0])
+
+    def adam_step(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
+                  eps: float = 1e-8) -> None:
+        check(lib().lct_cnn_adam_step(self.h, C.c_float(lr), C.c_float(beta1),
+                                      C.c_float(beta2), C.c_float(eps)))
