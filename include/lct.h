@@ -218,6 +218,13 @@ int lct_project_ellipsoids(lct_ctx *ctx, const lct_ct_geom *g, const float *h_an
  * reproducible for a given seed. */
 int lct_poisson_noise(lct_ctx *ctx, float *d_proj, size_t n, float i0, uint64_t seed);
 
+/* The same ellipsoids sampled at voxel centres x = (ix - wx) dx,
+ * y = (iy - wy) dy, z = z0 + iz dz (supersample^3 sub-samples averaged):
+ * d_vol [nz][ny][nx] f32 (ig->nz is ignored).  Training targets of the
+ * learned correction. */
+int lct_voxelize_ellipsoids(lct_ctx *ctx, const lct_img_geom *ig, double z0, int nz,
+                            const double *h_ell, int n_ell, int supersample, float *d_vol);
+
 /* Reprojection of a voxel volume d_vol [vol_nz][ny][nx] (voxel centres
  * x = (ix - wx) dx, y = (iy - wy) dy, z = vol_z0 + iz dz; nz of vg is
  * ignored) along the rays of lct_project_ellipsoids: trilinear samples at the

@@ -622,7 +622,8 @@ int lct_cnn_create(lct_ctx *ctx, uint64_t seed, lct_cnn **out)
 int lct_cnn_destroy(lct_cnn *n)
 {
     if (!n) return LCT_OK;
-    if (n->ctx) (void)hipStreamSynchronize(n->ctx->stream);
+    // the context may already be destroyed: wait on the device, not its stream
+    (void)hipDeviceSynchronize();
     for (void *p : {static_cast<void *>(n->w), static_cast<void *>(n->m), static_cast<void *>(n->v),
                     static_cast<void *>(n->g), static_cast<void *>(n->wb), static_cast<void *>(n->wt),
                     static_cast<void *>(n->wio), static_cast<void *>(n->loss),

@@ -113,9 +113,63 @@ __global__ __launch_bounds__(256) void poisson_noise_kernel(float *__restrict__ 
     p[i] = float(-log(fmax(cnt, 1.0) / i0));
 }
 
+// Phantom density at voxel centres (ss^3 sub-samples per voxel averaged).
+__global__ __launch_bounds__(256) void voxelize_kernel(const double *__restrict__ ell, int n_ell,
+                                                       int nx, int ny, int nz, double wx,
+                                                       double wy, double dx, double dy,
+                                                       double z0, double dz, int ss,
+                                                       float *__restrict__ vol)
+{
+    __shared__ double se[kMaxEll * kEllStride];
+    for (int i = threadIdx.x; i < n_ell * kEllStride; i += blockDim.x) se[i] = ell[i];
+    __syncthreads();
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y, iz = blockIdx.z;
+    if (ix >= nx) return;
+    double acc = 0.0;
+    for (int sz = 0; sz < ss; sz++)
+        for (int sy = 0; sy < ss; sy++)
+            for (int sx = 0; sx < ss; sx++) {
+                const double x = (double(ix) + (sx + 0.5) / ss - 0.5 - wx) * dx;
+                const double y = (double(iy) + (sy + 0.5) / ss - 0.5 - wy) * dy;
+                const double z = z0 + (double(iz) + (sz + 0.5) / ss - 0.5) * dz;
+                for (int e = 0; e < n_ell; e++) {
+                    const double *E = se + e * kEllStride;
+                    const double *R = E + 6;
+                    const double r0 = x - E[0], r1 = y - E[1], r2 = z - E[2];
+                    const double p0 = (R[0] * r0 + R[3] * r1 + R[6] * r2) / E[3];
+                    const double p1 = (R[1] * r0 + R[4] * r1 + R[7] * r2) / E[4];
+                    const double p2 = (R[2] * r0 + R[5] * r1 + R[8] * r2) / E[5];
+                    if (p0 * p0 + p1 * p1 + p2 * p2 <= 1.0) acc += E[15];
+                }
+            }
+    vol[(size_t(iz) * ny + iy) * nx + ix] = float(acc / double(ss * ss * ss));
+}
+
 }  // namespace lct
 
 using namespace lct;
+
+extern "C" int lct_voxelize_ellipsoids(lct_ctx *ctx, const lct_img_geom *ig, double z0, int nz,
+                                       const double *h_ell, int n_ell, int supersample,
+                                       float *d_vol)
+{
+    if (!ctx || !ig || !h_ell || !d_vol || nz <= 0 || ig->nx <= 0 || ig->ny <= 0 || n_ell < 0 ||
+        n_ell > kMaxEll || supersample < 1 || supersample > 8)
+        return set_error(LCT_ERR_INVALID, "lct_voxelize_ellipsoids: bad argument");
+    LCT_CHECK_HIP(hipSetDevice(ctx->device));
+    void *de = nullptr;
+    LCT_CHECK(scratch(ctx, kScratchMisc1, size_t(n_ell) * kEllStride * 8 + 8, &de));
+    LCT_CHECK(upload(ctx, de, h_ell, size_t(n_ell) * kEllStride * 8));
+    const dim3 grid(unsigned((ig->nx + 255) / 256), unsigned(ig->ny), unsigned(nz));
+    hipLaunchKernelGGL(voxelize_kernel, grid, dim3(256), 0, ctx->stream,
+                       static_cast<const double *>(de), n_ell, ig->nx, ig->ny, nz, double(ig->wx),
+                       double(ig->wy), double(ig->dx), double(ig->dy), z0, double(ig->dz),
+                       supersample, d_vol);
+    LCT_CHECK_HIP(hipGetLastError());
+    LCT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return LCT_OK;
+}
 
 extern "C" int lct_project_ellipsoids(lct_ctx *ctx, const lct_ct_geom *g, const float *h_angles,
                                       const float *h_couch, long view0, long n_views,

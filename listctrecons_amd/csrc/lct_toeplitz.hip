@@ -3,19 +3,22 @@
 //
 //   Q[(v,r)][ch] = sum_k T[ch][k] P[(v,r)][k],   T[ch][k] = h[ch - k + ns]
 //
-// C (M = channels) = A (M x K, Toeplitz taps) * B (K x N, N = every row of
-// every view).  Each FP32 operand x is split into bf16 hi = rn(x) and
-// lo = rn(x - hi); hi*hi + hi*lo + lo*hi (three v_mfma_f32_16x16x32_bf16 per
-// fragment, FP32 accumulation) keeps the products to ~2^-16 relative,
-// which is FP32-level for the filtered projections.  The result is written in
-// the backprojector's channel-major layout Q[view][slot_ch][slot_row]
-// (see lct_resident.hip) as f32, f16 or bf16.
+// C (M = channels) = A (M x K, the Toeplitz taps) * B (K x N, N = every row
+// of every view).  The taps are split three ways, h = h1 + h2 + h3 (bf16
+// each, exact to ~2^-25), the data two ways, x = x1 + x2 (~2^-17), and
+//   h1 x1 + h1 x2 + h2 x1 + h3 x1
+// is accumulated in FP32 with four v_mfma_f32_16x16x32_bf16 per fragment.
+// The tap split matters: its error is the same for every row, so it would
+// bias the filter's DC response; the data split's error is random per sample
+// and averages out in the backprojection.  The result is written in the
+// backprojector's channel-major layout Q[view][slot_ch][slot_row] (see
+// lct_resident.hip) as f32, f16 or bf16.
 //
-// A operand: a lane needs 8 consecutive taps h[i - k + ns] for k = k0..k0+7,
-// i.e. 8 consecutive entries of the reversed taps starting at an offset that
-// is not 16-byte aligned in general; the workgroup keeps 8 copies of the
-// reversed taps shifted by 0..7 elements in LDS so every fragment is one
-// aligned 16-byte read.
+// A fragments.  An A fragment (16 channels x 32 k) depends only on the
+// diagonal d = m0 - k0 (a multiple of 16), so every fragment the GEMM needs
+// is precomputed on the host in the MFMA register layout (lane l holds
+// T[m0 + l%16][k0 + 8(l/16) + j], j = 0..7) and read as one coalesced
+// 16-byte load per lane from an L2-resident table.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -37,12 +40,12 @@ constexpr int kBRow = 40;    // LDS row of one B column: 32 k + 8 pad (bf16)
 constexpr int kRowPadT = 4;  // zero row slots before row 0 in Q (as lct_resident.hip)
 
 struct ToeplitzArgs {
-    const unsigned short *taps_hi;  // [8][L] shifted reversed taps (bf16 bits)
-    const unsigned short *taps_lo;
-    const float *in;                // P [(v,r)][ns] f32
-    void *q;                        // Q [v][cp][rp]
-    long n_cols;                    // V * nt
-    int ns, nt, cp, rp, kp, L, P, n_m_tiles, n_n_wg;
+    const unsigned short *frag;  // [3][n_diag][64 lanes][8] bf16 bits
+    long frag_stride;            // n_diag * 64 * 8
+    const float *in;             // P [(v,r)][ns] f32
+    void *q;                     // Q [v][cp][rp]
+    long n_cols;                 // V * nt
+    int ns, nt, cp, rp, kp, n_m_tiles, n_n_wg;
 };
 
 __device__ __forceinline__ unsigned short bf16_rn(float x)
@@ -65,38 +68,21 @@ template <> __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(f
 template <typename T>
 __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
 {
-    extern __shared__ __attribute__((aligned(16))) unsigned short smem[];
-    unsigned short *th = smem;                 // [8][L]
-    unsigned short *tl = th + 8 * A.L;         // [8][L]
-    unsigned short *bh = tl + 8 * A.L;         // [2][kTN][kBRow]
-    unsigned short *bl = bh + 2 * kTN * kBRow; // [2][kTN][kBRow]
+    __shared__ __attribute__((aligned(16))) unsigned short bh[2 * kTN * kBRow];
+    __shared__ __attribute__((aligned(16))) unsigned short bl[2 * kTN * kBRow];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int mt = blockIdx.x % A.n_m_tiles;
     const int nw = blockIdx.x / A.n_m_tiles;
     const int m0 = mt * kTM;
-
-    // taps: 16-byte copies from global (L2 resident)
-    {
-        const uint4 *sh = reinterpret_cast<const uint4 *>(A.taps_hi);
-        const uint4 *sl = reinterpret_cast<const uint4 *>(A.taps_lo);
-        uint4 *dh = reinterpret_cast<uint4 *>(th);
-        uint4 *dl = reinterpret_cast<uint4 *>(tl);
-        const int n16 = 8 * A.L / 8;
-        for (int i = tid; i < n16; i += 256) {
-            dh[i] = sh[i];
-            dl[i] = sl[i];
-        }
-    }
-
-    const int li = lane & 15;       // fragment row / column within 16
-    const int lk = (lane >> 4) * 8; // k offset of this lane's 8 values
+    const int li = lane & 15;
+    const int lk = (lane >> 4) * 8;
     const int n_ksteps = A.kp / kTK;
 
     // B staging: thread -> column tid / 4, k group (tid % 4) * 8
     const int bcol = tid >> 2, bk = (tid & 3) * 8;
-    auto stage_b = [&](long n0, int ks, int buf, float (&x)[8]) {
+    auto stage_b = [&](long n0, int ks, float (&x)[8]) {
         const long col = n0 + bcol;
         const int k = ks * kTK + bk;
         if (col < A.n_cols && k < A.ns) {
@@ -114,7 +100,6 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
 #pragma unroll
             for (int j = 0; j < 8; j++) x[j] = 0.0f;
         }
-        (void)buf;
     };
     auto commit_b = [&](int buf, const float (&x)[8]) {
         unsigned short hi[8], lo[8];
@@ -132,8 +117,6 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
         *reinterpret_cast<uint4 *>(bl + (buf * kTN + bcol) * kBRow + bk) = vl;
     };
 
-    __syncthreads();  // taps in LDS
-
     const long n_tiles = (A.n_cols + kTN - 1) / kTN;
     for (long nt_i = nw; nt_i < n_tiles; nt_i += A.n_n_wg) {
         const long n0 = nt_i * kTN;
@@ -144,21 +127,23 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
             for (int f = 0; f < 4; f++) acc[a][f] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
         float xb[8];
-        stage_b(n0, 0, 0, xb);
+        stage_b(n0, 0, xb);
         commit_b(0, xb);
         __syncthreads();
         for (int ks = 0; ks < n_ksteps; ks++) {
             const int buf = ks & 1;
-            if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, buf ^ 1, xb);
-            // A fragments of this wave's two 16-channel slices
-            bf16x8 ah[2], al[2];
+            if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, xb);
+            // A fragments of this wave's two 16-channel slices: diagonal index
+            // (m - k0 + kp - 32) / 16 of the precomputed table
+            bf16x8 a1[2], a2[2], a3[2];
 #pragma unroll
             for (int a = 0; a < 2; a++) {
-                const int i = m0 + (2 * wave + a) * 16 + li;   // output channel
-                const int o = A.ns - 1 - i + ks * kTK + lk + A.P;  // >= 0 by the choice of P
-                const int s = o & 7, u = o - s;
-                ah[a] = *reinterpret_cast<const bf16x8 *>(th + s * A.L + u);
-                al[a] = *reinterpret_cast<const bf16x8 *>(tl + s * A.L + u);
+                const int m = m0 + (2 * wave + a) * 16;
+                const long di = (m - ks * kTK + A.kp - kTK) / 16;
+                const unsigned short *f = A.frag + (di * 64 + lane) * 8;
+                a1[a] = *reinterpret_cast<const bf16x8 *>(f);
+                a2[a] = *reinterpret_cast<const bf16x8 *>(f + A.frag_stride);
+                a3[a] = *reinterpret_cast<const bf16x8 *>(f + 2 * A.frag_stride);
             }
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -167,9 +152,10 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
                 const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(bl + (buf * kTN + c) * kBRow + lk);
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], b_hi, acc[a][f], 0, 0, 0);
-                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], b_lo, acc[a][f], 0, 0, 0);
-                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], b_hi, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[a], b_hi, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[a], b_hi, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_lo, acc[a][f], 0, 0, 0);
+                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_hi, acc[a][f], 0, 0, 0);
                 }
             }
             if (ks + 1 < n_ksteps) commit_b(buf ^ 1, xb);
@@ -204,7 +190,7 @@ static unsigned short host_bf16_rn(float x)
     return (unsigned short)(u >> 16);
 }
 
-static float host_bf16_f(unsigned short b)
+static double host_bf16_f(unsigned short b)
 {
     unsigned u = unsigned(b) << 16;
     float f;
@@ -218,29 +204,36 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
 {
     const int mp = (ns + kTM - 1) / kTM * kTM;
     const int kp = (ns + kTK - 1) / kTK * kTK;
-    const int P = ((std::max(mp - ns, 0) + 7) / 8) * 8;
-    const int L = ((ns + kp + P + 16 + 7) / 8) * 8;
-    // reversed taps hr[t] = h[2 ns - 1 - t], padded by P, in 8 shifted copies
-    std::vector<float> hr(size_t(L) + 8, 0.0f);
-    for (int t = 0; t < 2 * ns; t++) hr[size_t(t) + P] = h_taps[2 * ns - 1 - t];
-    std::vector<unsigned short> hi(8 * size_t(L)), lo(8 * size_t(L));
-    for (int s = 0; s < 8; s++)
-        for (int u = 0; u < L; u++) {
-            const float x = hr[size_t(u) + s];
-            const unsigned short b = host_bf16_rn(x);
-            hi[size_t(s) * L + u] = b;
-            lo[size_t(s) * L + u] = host_bf16_rn(x - host_bf16_f(b));
-        }
+    // diagonals d = m - k0 from -(kp - 32) to mp - 16 in steps of 16
+    const int n_diag = (mp + kp - kTK) / 16;
+    const long stride = long(n_diag) * 64 * 8;
+    std::vector<unsigned short> frag(3 * size_t(stride));
+    for (int di = 0; di < n_diag; di++) {
+        const int d = di * 16 - (kp - kTK);
+        for (int lane = 0; lane < 64; lane++)
+            for (int j = 0; j < 8; j++) {
+                // T[m + i][k0 + kk] = h[d + i - kk + ns]
+                const int i = lane & 15, kk = (lane >> 4) * 8 + j;
+                const int idx = d + i - kk + ns;
+                const double h = (idx >= 0 && idx < 2 * ns) ? double(h_taps[idx]) : 0.0;
+                const unsigned short b1 = host_bf16_rn(float(h));
+                const double r1 = h - host_bf16_f(b1);
+                const unsigned short b2 = host_bf16_rn(float(r1));
+                const double r2 = r1 - host_bf16_f(b2);
+                const unsigned short b3 = host_bf16_rn(float(r2));
+                const size_t o = (size_t(di) * 64 + lane) * 8 + j;
+                frag[o] = b1;
+                frag[stride + o] = b2;
+                frag[2 * stride + o] = b3;
+            }
+    }
     void *dt = nullptr;
-    LCT_CHECK(scratch(ctx, kScratchTaps, hi.size() * 4, &dt));
-    unsigned short *d_hi = static_cast<unsigned short *>(dt);
-    unsigned short *d_lo = d_hi + hi.size();
-    LCT_CHECK(upload(ctx, d_hi, hi.data(), hi.size() * 2));
-    LCT_CHECK(upload(ctx, d_lo, lo.data(), lo.size() * 2));
+    LCT_CHECK(scratch(ctx, kScratchTaps, frag.size() * 2, &dt));
+    LCT_CHECK(upload(ctx, dt, frag.data(), frag.size() * 2));
 
     ToeplitzArgs a;
-    a.taps_hi = d_hi;
-    a.taps_lo = d_lo;
+    a.frag = static_cast<const unsigned short *>(dt);
+    a.frag_stride = stride;
     a.in = d_in;
     a.q = d_q;
     a.n_cols = n_cols;
@@ -249,18 +242,11 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
     a.cp = cp;
     a.rp = rp;
     a.kp = kp;
-    a.L = L;
-    a.P = P;
     a.n_m_tiles = mp / kTM;
     const long n_tiles = (n_cols + kTN - 1) / kTN;
-    a.n_n_wg = int(std::min<long>(n_tiles, std::max<long>(1, 2048 / a.n_m_tiles)));
-    const size_t shm = (size_t(16) * L + size_t(4) * kTN * kBRow) * sizeof(unsigned short);
-    if (shm > 160 * 1024) return set_error(LCT_ERR_UNSUPPORTED, "toeplitz filter: ns too large");
-    if (shm > 64 * 1024)
-        LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&toeplitz_filter_kernel<T>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
+    a.n_n_wg = int(std::min<long>(n_tiles, std::max<long>(1, 4096 / a.n_m_tiles)));
     hipLaunchKernelGGL(toeplitz_filter_kernel<T>, dim3(unsigned(a.n_m_tiles * a.n_n_wg)), dim3(256),
-                       shm, ctx->stream, a);
+                       0, ctx->stream, a);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import dataclasses
+import sys
 
 import numpy as np
 
@@ -139,6 +140,10 @@ class Context:
         self.close()
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down: the
+        # process releases device memory itself
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -222,6 +227,13 @@ class Context:
                                         _dev(vol), C.byref(ig), C.c_float(z0), vol.shape[0],
                                         C.c_float(step), _dev(out)))
 
+    def voxelize(self, ig: _lib.ImgGeom, z0: float, nz: int, ell: np.ndarray, out,
+                 supersample: int = 2) -> None:
+        e = np.ascontiguousarray(ell, np.float64).reshape(-1, 16)
+        check(lib().lct_voxelize_ellipsoids(self.h, C.byref(ig), float(z0), nz,
+                                            e.ctypes.data_as(C.POINTER(C.c_double)), e.shape[0],
+                                            supersample, _dev(out)))
+
     def poisson_noise(self, proj, i0: float, seed: int) -> None:
         check(lib().lct_poisson_noise(self.h, _dev(proj), proj.numel(), C.c_float(i0),
                                       C.c_uint64(seed)))
@@ -267,6 +279,10 @@ class Cnn:
             self.h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down: the
+        # process releases device memory itself
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

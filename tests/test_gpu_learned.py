@@ -1,0 +1,63 @@
+"""The learned correction end to end at pitch 1.5: training pairs made by the
+method itself, the reprojection-residual gate, and a short training run that
+must lower the RMSE to the phantom on held-out scans."""
+import numpy as np
+import pytest
+import torch
+
+from listctrecons_amd import learned, recon, scan
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def w():
+    return learned.learned_workload()
+
+
+@pytest.fixture(scope="module")
+def cases(ctx, w):
+    print("making training and held-out cases", flush=True)
+    train = [learned.make_case(ctx, w, seed) for seed in range(1000, 1010)]
+    held = [learned.make_case(ctx, w, seed) for seed in (5000, 5001)]
+    return train, held
+
+
+def test_case_targets_and_fdk(ctx, w, cases):
+    train, held = cases
+    c = train[0]
+    assert c.fdk.shape == c.target.shape == (128, 256, 256)
+    # the target is RECON_SCALE x a phantom with densities in [0, 0.025] (overlaps add)
+    assert float(c.target.min()) >= 0.0
+    r = learned.rmse_to_phantom(c.fdk, c)
+    print(f"FDK rmse to phantom {r:.3e} /mm ({r / scan.MU_WATER * 1000:.1f} HU)", flush=True)
+    assert r < 3e-3
+
+
+def test_reprojection_gate_rejects_a_worse_volume(ctx, w, cases):
+    train, held = cases
+    c = held[0]
+    r_fdk = learned.reprojection_residual(ctx, c.fdk, c, w)
+    r_bad = learned.reprojection_residual(ctx, c.fdk * 1.1, c, w)
+    r_true = learned.reprojection_residual(ctx, c.target, c, w)
+    print(f"residuals: fdk {r_fdk:.4f} scaled {r_bad:.4f} phantom {r_true:.4f}", flush=True)
+    assert r_bad > r_fdk
+    assert r_true < 0.05  # the voxelised phantom reprojects to within noise + voxelisation
+
+
+def test_training_lowers_rmse_on_held_out_scans(ctx, w, cases):
+    train, held = cases
+    net = recon.Cnn(ctx, seed=11)
+    losses = learned.train(ctx, net, train, steps=300, batch=8, size=64, lr=1e-3, seed=3,
+                           log=lambda s: print(s, flush=True))
+    assert np.mean(losses[-20:]) < 0.7 * np.mean(losses[:5])
+    gains = []
+    for c in held:
+        vol, r_fdk, r_cor = learned.gated(ctx, net, c, w)
+        e_fdk = learned.rmse_to_phantom(c.fdk, c)
+        e_cor = learned.rmse_to_phantom(learned.correct(net, c.fdk), c)
+        gains.append(1 - e_cor / e_fdk)
+        print(f"held-out rmse fdk {e_fdk:.3e} corrected {e_cor:.3e} gain {gains[-1]:.1%}; "
+              f"reprojection fdk {r_fdk:.4f} corrected {r_cor:.4f}", flush=True)
+    assert min(gains) > 0.10
+    net.close()

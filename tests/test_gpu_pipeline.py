@@ -41,7 +41,7 @@ def cfg1():
 
 
 @pytest.mark.parametrize("preset,tol", [("plain", 1e-4), ("fast_f32", 1e-4),
-                                        ("fast_f32_gather", 1e-4), ("fast_f32_toeplitz", 1e-4)])
+                                        ("fast_f32_gather", 1e-4), ("fast_f32_toeplitz", 2e-4)])
 def test_fp32_paths_match_oracle(ctx, cfg1, preset, tol):
     w, fan, ref = cfg1
     print(f"{preset} on {w.name}", flush=True)
@@ -136,7 +136,8 @@ def sampled(vol, refs):
     return np.stack(g), np.stack(r)
 
 
-@pytest.mark.parametrize("preset", ["plain", "fast_f32", "mixed_f16", "mixed_bf16"])
+@pytest.mark.parametrize("preset", ["plain", "fast_f32", "resident_direct_f16", "mixed_f16",
+                                    "mixed_bf16"])
 def test_clinical_sampled_parity(ctx, clinical, clinical_ref, preset):
     w, fan, a, c = clinical
     print(f"{preset} on {w.name}", flush=True)

@@ -82,7 +82,7 @@ def test_filter(ctx, data, name, window):
 
 
 @pytest.mark.parametrize("name", CASES)
-@pytest.mark.parametrize("algo,storage,tol", [(0, 0, 2e-6), (1, 0, 2e-5), (1, 1, 1e-3),
+@pytest.mark.parametrize("algo,storage,tol", [(0, 0, 2e-6), (1, 0, 1e-4), (1, 1, 1e-3),
                                               (1, 2, 5e-3), (0, 1, 1e-3)])
 def test_filter_into_backprojector_layout(ctx, data, name, algo, storage, tol):
     """Direct and BF16x3 Toeplitz-MFMA filters, Q[view][ns+3][nt+8] output."""
