@@ -114,6 +114,11 @@ def main():
     ap.add_argument("--path", default="mixed_f16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--variants", default="",
+                    help="comma-separated presets to time besides the headline path "
+                         "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
+    ap.add_argument("--copy-probe-gib", type=float, default=0.0,
+                    help="run one device copy of this many GiB first (PMC scale reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the helical FDK path shards by "
@@ -127,6 +132,16 @@ def main():
     from listctrecons_amd import recon, scan
 
     w = scan.workload(args.workload)
+    if args.copy_probe_gib > 0:
+        # a plain device-to-device copy of known size: the HBM counters of this
+        # dispatch calibrate the FETCH_SIZE / WRITE_SIZE readings of the others
+        n = int(args.copy_probe_gib * (1 << 30)) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
     ctx = recon.Context(0)
     log(f"synthesising {w.name}: {w.readings} views of {w.nt}x{w.ns}")
     fan, a, c = recon.synth_scan(ctx, w)
@@ -150,8 +165,22 @@ def main():
         plain_vol = vol2.cpu().numpy()
         log(f"  {plain['s_per_volume']*1e3:.2f} ms/volume  stages {plain['stage_ms']}")
 
-    storage = {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16"}.get(
-        args.path, "f32")
+    variants = {}
+    for name in [v for v in args.variants.split(",") if v]:
+        log(f"timing variant {name}")
+        vv = torch.empty_like(vol)
+        r = time_path(ctx, pl, recon.preset(name, window=w.window), a, c, fan, vv,
+                      max(1, min(args.steps, 3)), 1)
+        d = (vv.cpu().numpy() - head_vol).astype(np.float64) * scan.hu_per_unit()
+        variants[name] = {"s_per_volume": round(r["s_per_volume"], 6),
+                          "stage_ms": {k: round(x, 3) for k, x in r["stage_ms"].items()},
+                          "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"]
+                                                                    * 1e-3) / 1e9, 2),
+                          "hu_rmse_vs_headline": round(float(np.sqrt(np.mean(d * d))), 4)}
+        del vv
+
+    storage = {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16",
+               "mixed_f16_gather": "f16", "mixed_bf16_gather": "bf16"}.get(args.path, "f32")
     bytes_per_tap = 2 if storage in ("f16", "bf16") else 4
     updates = head["updates"]
     bp_s = head["bp_ms_per_launch"] * 1e-3
@@ -193,6 +222,8 @@ def main():
                           f"per launch",
         },
     }
+    if variants:
+        out["variants"] = variants
     if plain:
         out["fp32_plain_s_per_volume"] = round(plain["s_per_volume"], 6)
         out["fp32_plain_stage_ms"] = {k: round(v, 3) for k, v in plain["stage_ms"].items()}

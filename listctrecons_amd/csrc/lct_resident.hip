@@ -178,12 +178,14 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
     return s > cp - W ? cp - W : s;
 }
 
-template <typename T, int NPRE>
+// RP > 0: row stride (nt + 8) known at compile time, so the two channels of
+// a bilinear tap are read with immediate LDS offsets from one address.
+template <typename T, int NPRE, int RP>
 __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 {
     extern __shared__ float lds[];
     constexpr int CE = Chunk<T>::kElems;
-    const int W = A.W, rp = A.rp;
+    const int W = A.W, rp = RP > 0 ? RP : A.rp;
     const int view_elems = W * rp;          // one view's window
     const int stage = 2 * view_elems;       // views j and j+h
     const int n_chunks = stage / CE;
@@ -281,8 +283,12 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const float tz = theta * A.inv_orbit;
         const float zr = A.z0 + (float(j) + tz) * A.dzv;
         const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
-        const float k1 = A.kc / lhat, k2 = A.kc / lhat2;
+        const float k1 = A.kc * __builtin_amdgcn_rcpf(lhat), k2 = A.kc * __builtin_amdgcn_rcpf(lhat2);
         const float o1 = -zr * k1, o2 = -zr2 * k2;
+        // row coordinate in slot units (row + kRowPad), clamped to the guard
+        // slots [kRowPad - 1, nt + kRowPad]: positive, so truncation is floor
+        const float row0 = A.wt + float(kRowPad);
+        const float lo_r = float(kRowPad) - 1.0f, hi_r = A.nt_f + float(kRowPad);
 #pragma unroll
         for (int z = 0; z < 32; z++) {
             if (am[z] > j && am[z] <= j + A.h) {  // pair inside this slice's window
@@ -291,20 +297,21 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
                 const float qh2 = fmaf(zc, k2, o2);
                 const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
                 const float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
-                float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
-                float w2 = out1 ? (out2 ? 0.5f : 1.0f) : (out2 ? 0.0f : 1.0f - wb);
-                const float r1 = fminf(fmaxf(qh + A.wt, -1.0f), A.nt_f);
-                const float r2 = fminf(fmaxf(qh2 + A.wt, -1.0f), A.nt_f);
-                const float fr1 = floorf(r1), fr2 = floorf(r2);
-                const float ar1 = r1 - fr1, ar2 = r2 - fr2;
-                const int i1 = int(fr1) + kRowPad, i2 = int(fr2) + kRowPad;
-                const float a00 = b1[i1], a01 = b1[i1 + 1], a10 = b1[rp + i1], a11 = b1[rp + i1 + 1];
-                const float c00 = b2[i2], c01 = b2[i2 + 1], c10 = b2[rp + i2], c11 = b2[rp + i2 + 1];
+                // conjugate weights (w1, 1 - w1); both rays off the detector: 1/2 each
+                const float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
+                const float r1 = __builtin_amdgcn_fmed3f(qh + row0, lo_r, hi_r);
+                const float r2 = __builtin_amdgcn_fmed3f(qh2 + row0, lo_r, hi_r);
+                const int i1 = int(r1), i2 = int(r2);
+                const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
+                const float *p1 = b1 + i1;
+                const float *p2 = b2 + i2;
+                const float a00 = p1[0], a01 = p1[1], a10 = p1[rp], a11 = p1[rp + 1];
+                const float c00 = p2[0], c01 = p2[1], c10 = p2[rp], c11 = p2[rp + 1];
                 const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
                 const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
                 const float t1 = fmaf(ar1, ha - la, la);
                 const float t2 = fmaf(ar2, hc - lc, lc);
-                acc[z] = fmaf(w1, t1, fmaf(w2, t2, acc[z]));
+                acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
             }
         }
         if (j + 1 < jhi) commit(lds + (cur ^ 1) * stage);
@@ -423,28 +430,24 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     if (shm > 160 * 1024)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
     const int npre = (chunks + 255) / 256;
-#define LCT_BP_CASE(N)                                                                      \
-    case N:                                                                                 \
-        if (shm > 64 * 1024)                                                                \
-            LCT_CHECK_HIP(hipFuncSetAttribute(                                              \
-                reinterpret_cast<const void *>(&bp_fold_kernel<T, N>),                      \
-                hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));                     \
-        hipLaunchKernelGGL((bp_fold_kernel<T, N>), dim3(unsigned(a.n_wg)), dim3(256), shm,  \
-                           ctx->stream, a);                                                 \
-        break;
-    switch (npre) {
-        LCT_BP_CASE(1)
-        LCT_BP_CASE(2)
-        LCT_BP_CASE(3)
-        LCT_BP_CASE(4)
-        LCT_BP_CASE(5)
-        LCT_BP_CASE(6)
-        LCT_BP_CASE(7)
-        LCT_BP_CASE(8)
-    default:
+    if (npre < 1 || npre > 8)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window too large");
-    }
-#undef LCT_BP_CASE
+    // row strides of the 16-, 64- and 96-row detectors get immediate LDS offsets
+    using Kern = void (*)(FoldBpArgs);
+    static const Kern table[4][8] = {
+#define LCT_ROW(RP)                                                                              \
+    {bp_fold_kernel<T, 1, RP>, bp_fold_kernel<T, 2, RP>, bp_fold_kernel<T, 3, RP>,               \
+     bp_fold_kernel<T, 4, RP>, bp_fold_kernel<T, 5, RP>, bp_fold_kernel<T, 6, RP>,               \
+     bp_fold_kernel<T, 7, RP>, bp_fold_kernel<T, 8, RP>}
+        LCT_ROW(0), LCT_ROW(24), LCT_ROW(72), LCT_ROW(104)
+#undef LCT_ROW
+    };
+    const int row = a.rp == 24 ? 1 : a.rp == 72 ? 2 : a.rp == 104 ? 3 : 0;
+    const Kern k = table[row][npre - 1];
+    if (shm > 64 * 1024)
+        LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
+    hipLaunchKernelGGL(k, dim3(unsigned(a.n_wg)), dim3(256), shm, ctx->stream, a);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -458,7 +461,8 @@ static int tile_half_width(const lct_ct_geom &g, const lct_img_geom &ig)
     double u = std::sqrt(rx * rx + ry * ry) / g.dso;
     u = std::min(u, 0.999);
     const double slope = 1.0 / std::sqrt(1.0 - u * u);
-    const double half_extent = 0.5 * (kTile - 1) * (ig.dx + ig.dy);  // |dx cos| + |dy sin| bound
+    // |dx cos b + dy sin b| <= sqrt(dx^2 + dy^2) for the column offsets of a tile
+    const double half_extent = 0.5 * (kTile - 1) * std::sqrt(double(ig.dx) * ig.dx + double(ig.dy) * ig.dy);
     return int(std::ceil(half_extent * slope / g.ds)) + 2;
 }
 

@@ -156,3 +156,98 @@ def gated(ctx, net, case: Case, w: scan.Workload):
     r_fdk = reprojection_residual(ctx, case.fdk, case, w)
     r_cor = reprojection_residual(ctx, corr, case, w)
     return (corr if r_cor <= r_fdk else case.fdk), r_fdk, r_cor
+
+
+def run(train_phantoms: int = 2000, pool: int = 96, steps: int = 2000, refresh_every: int = 10,
+        held_out: int = 2, seed: int = 0, log=print) -> dict:
+    """The pitch-1.5 learned-correction experiment.
+
+    Training: `train_phantoms` distinct seeded phantoms (seeds 1000...) are
+    streamed through a pool of `pool` FDK/phantom pairs at 256 x 256 x 128,
+    one pair replaced every `refresh_every` steps after the pool is full;
+    batch 8 of 64^3 patches, Adam lr 1e-3.
+    Evaluation: `held_out` unseen phantoms (seeds 9000...) reconstructed at
+    512 x 512 x 256 (0.8 mm voxels, the training voxel size: the 512^2 grid
+    spans the 208 mm scan field of view with margin), RMSE to the phantom in
+    the field of view for FDK and for the gated correction, and both
+    reprojection residuals."""
+    import torch
+
+    ctx = recon.Context(0)
+    w = learned_workload()
+    w_eval = learned_workload(readings=9900, nx=512)
+    t0 = time.perf_counter()
+    cases = [make_case(ctx, w, 1000 + i) for i in range(min(pool, train_phantoms))]
+    gen_s = time.perf_counter() - t0
+    next_seed = 1000 + len(cases)
+    log(f"pool of {len(cases)} pairs in {gen_s:.1f} s")
+    net = recon.Cnn(ctx, seed=seed)
+    rng = np.random.default_rng(seed)
+    losses = []
+    t_train = time.perf_counter()
+    for s in range(steps):
+        if (s + 1) % refresh_every == 0 and next_seed < 1000 + train_phantoms:
+            cases[int(rng.integers(len(cases)))] = make_case(ctx, w, next_seed)
+            next_seed += 1
+        x, t = random_patches(cases, 8, 64, rng)
+        losses.append(net.loss_grad(x, t))
+        net.adam_step(1e-3)
+        if s % 100 == 0 or s == steps - 1:
+            log(f"step {s}: loss {losses[-1]:.5g}  phantoms seen {next_seed - 1000}  "
+                f"({time.perf_counter() - t_train:.0f} s)")
+    ctx.synchronize()
+    train_s = time.perf_counter() - t_train
+    del cases
+    torch.cuda.empty_cache()
+
+    evals = []
+    for k in range(held_out):
+        c = make_case(ctx, w_eval, 9000 + k)
+        torch.cuda.synchronize()
+        ti = time.perf_counter()
+        corr = correct(net, c.fdk)
+        torch.cuda.synchronize()
+        infer_s = time.perf_counter() - ti
+        r_fdk = reprojection_residual(ctx, c.fdk, c, w_eval)
+        r_cor = reprojection_residual(ctx, corr, c, w_eval)
+        e_fdk = rmse_to_phantom(c.fdk, c)
+        e_cor = rmse_to_phantom(corr, c)
+        accepted = r_cor <= r_fdk
+        evals.append({"seed": 9000 + k, "rmse_fdk_per_mm": e_fdk, "rmse_corrected_per_mm": e_cor,
+                      "rmse_gain": 1 - e_cor / e_fdk,
+                      "rmse_fdk_hu": e_fdk / scan.MU_WATER * 1000,
+                      "rmse_corrected_hu": e_cor / scan.MU_WATER * 1000,
+                      "reprojection_residual_fdk": r_fdk, "reprojection_residual_corrected": r_cor,
+                      "gate_accepts_correction": accepted, "inference_s": infer_s})
+        log(f"held-out {9000 + k}: rmse {e_fdk:.3e} -> {e_cor:.3e} ({1 - e_cor / e_fdk:.1%}), "
+            f"reprojection {r_fdk:.4f} -> {r_cor:.4f}, accepted={accepted}, "
+            f"inference {infer_s * 1e3:.0f} ms")
+        del c, corr
+        torch.cuda.empty_cache()
+    net.close()
+    ctx.close()
+    return {"workload": "pitch 1.5, 16x368 detector, 576 views/turn",
+            "train_volume": "256x256x128 @ 0.8 mm", "eval_volume": "512x512x256 @ 0.8 mm",
+            "train_phantoms": next_seed - 1000, "pool": pool, "steps": steps, "batch": 8,
+            "patch": 64, "lr": 1e-3, "pool_generation_s": gen_s, "train_s": train_s,
+            "loss_first": float(np.mean(losses[:10])), "loss_last": float(np.mean(losses[-50:])),
+            "held_out": evals}
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-phantoms", type=int, default=2000)
+    ap.add_argument("--pool", type=int, default=96)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--refresh-every", type=int, default=1)
+    ap.add_argument("--held-out", type=int, default=2)
+    ap.add_argument("--out", default="")
+    args = ap.parse_args()
+    res = run(args.train_phantoms, args.pool, args.steps, args.refresh_every, args.held_out)
+    text = json.dumps(res, indent=1)
+    print(text, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
