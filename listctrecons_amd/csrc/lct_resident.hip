@@ -140,8 +140,13 @@ struct FoldBpArgs {
     const void *q;
     size_t view_stride;  // cp * rp elements
     int rp, cp, W, H;    // channel window of W = 2H + 2 slots
+    int lp;              // LDS row pitch of a staged channel (rp + 1: odd, so the
+                         // channels that neighbouring lanes read fall in different banks)
     const float *cosb, *sinb;  // indexed by Q view
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
+    const unsigned *zmask;     // per pair j - (a_middle[0] - h): bit z set when
+                               // a_middle[z] - h <= j < a_middle[z] (the pair
+                               // lies in slice z's window); the same for every block
     float *vol;
     int n_tiles_x, n_tiles_y, n_wg, xcd_remap;
     int nx, ny, h;
@@ -153,19 +158,27 @@ template <typename T> struct Chunk {
     static constexpr int kElems = 16 / int(sizeof(T));  // elements per 16-byte load
 };
 
-// Converts a 16-byte chunk of T to floats in LDS.
+// Converts a 16-byte chunk of T to floats at a 4-byte aligned LDS address
+// (two 16-bit values per 32-bit word, low half first).
 template <typename T>
-__device__ __forceinline__ void chunk_to_lds(float *dst, const uint4 &v)
+__device__ __forceinline__ void chunk_to_lds_odd(float *dst, const uint4 &v)
 {
     if constexpr (sizeof(T) == 4) {
-        reinterpret_cast<float4 *>(dst)[0] = make_float4(__uint_as_float(v.x), __uint_as_float(v.y),
-                                                         __uint_as_float(v.z), __uint_as_float(v.w));
+        dst[0] = __uint_as_float(v.x);
+        dst[1] = __uint_as_float(v.y);
+        dst[2] = __uint_as_float(v.z);
+        dst[3] = __uint_as_float(v.w);
     } else {
-        // two 16-bit values per 32-bit word, low half first
         auto lo = [](unsigned w) { return Cvt<T>::from(__builtin_bit_cast(T, (unsigned short)(w & 0xffffu))); };
         auto hi = [](unsigned w) { return Cvt<T>::from(__builtin_bit_cast(T, (unsigned short)(w >> 16))); };
-        reinterpret_cast<float4 *>(dst)[0] = make_float4(lo(v.x), hi(v.x), lo(v.y), hi(v.y));
-        reinterpret_cast<float4 *>(dst)[1] = make_float4(lo(v.z), hi(v.z), lo(v.w), hi(v.w));
+        dst[0] = lo(v.x);
+        dst[1] = hi(v.x);
+        dst[2] = lo(v.y);
+        dst[3] = hi(v.y);
+        dst[4] = lo(v.z);
+        dst[5] = hi(v.z);
+        dst[6] = lo(v.w);
+        dst[7] = hi(v.w);
     }
 }
 
@@ -178,17 +191,22 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
     return s > cp - W ? cp - W : s;
 }
 
-// RP > 0: row stride (nt + 8) known at compile time, so the two channels of
-// a bilinear tap are read with immediate LDS offsets from one address.
-template <typename T, int NPRE, int RP>
+// The staged windows are held in LDS as floats with an odd channel pitch lp
+// (A.lp = rp + 1): neighbouring voxel columns read neighbouring channels, and
+// an odd pitch spreads those reads over all LDS banks (the HBM pitch rp is a
+// multiple of 8, which would put them in a quarter of the banks).  LP > 0:
+// lp known at compile time, so the two channels of a bilinear tap are read
+// with immediate LDS offsets from one address.
+template <typename T, int NPRE, int LP>
 __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 {
     extern __shared__ float lds[];
     constexpr int CE = Chunk<T>::kElems;
-    const int W = A.W, rp = RP > 0 ? RP : A.rp;
-    const int view_elems = W * rp;          // one view's window
-    const int stage = 2 * view_elems;       // views j and j+h
-    const int n_chunks = stage / CE;
+    const int W = A.W, rp = A.rp, lp = LP > 0 ? LP : A.lp;
+    const int view_elems = W * rp;          // one view's window in Q
+    const int lds_view = W * lp;            // and in LDS
+    const int stage = 2 * lds_view;         // views j and j+h
+    const int n_chunks = 2 * view_elems / CE;
 
     // XCD-aware order: consecutive workgroups are dealt round-robin to the 8
     // XCDs; give each XCD a contiguous range of tiles so the views it stages
@@ -203,9 +221,6 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     const int iy = ty0 + int(threadIdx.x / kTile);
     const int *st = A.slab_tab + blk * 34;
     const int qoff = st[0], slice0 = st[1];
-    int am[32];
-#pragma unroll
-    for (int z = 0; z < 32; z++) am[z] = st[2 + z];
 
     const float xc = (float(ix) - A.wx) * A.dx;
     const float yc = (float(iy) - A.wy) * A.dy;
@@ -217,11 +232,12 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 #pragma unroll
     for (int z = 0; z < 32; z++) acc[z] = 0.0f;
 
-    const int jlo = am[0] - A.h, jhi = am[31];
+    // pairs of any slice's window: a_middle[0] - h <= j < a_middle[31]
+    const int jlo = st[2] - A.h, jhi = st[33];
 
     uint4 pre[NPRE];
-    auto win_of = [&](int j, int &s1, int &s2) {
-        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
+    // window of the pair whose first view has tube angle cos c, sin s
+    auto win_of = [&](float c, float s, int &s1, int &s2) {
         const float ph = -xcc * c - ycc * s;
         const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
         s1 = window_start(c1, A.H, A.cp, W);
@@ -240,30 +256,58 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
             }
         }
     };
+    // LDS offset of each chunk this thread stages (a chunk never straddles a
+    // channel: rp is a multiple of 8)
+    int dst[NPRE];
+#pragma unroll
+    for (int i = 0; i < NPRE; i++) {
+        const int e = (int(threadIdx.x) + i * 256) * CE;
+        const int v = e < view_elems ? 0 : 1;
+        const int ev = e - v * view_elems;
+        dst[i] = v * lds_view + (ev / rp) * lp + ev % rp;
+    }
     auto commit = [&](float *buf) {
 #pragma unroll
         for (int i = 0; i < NPRE; i++) {
             const int c = int(threadIdx.x) + i * 256;
-            if (c < n_chunks) chunk_to_lds<T>(buf + c * CE, pre[i]);
+            if (c < n_chunks) chunk_to_lds_odd<T>(buf + dst[i], pre[i]);
         }
     };
 
+    // Per-pair scalars are loaded ahead so no pair waits on them: cos/sin of
+    // pair j + 1's first view during pair j - 1 (pair j stages j + 1's window),
+    // the conjugate view's cos/sin and the slice mask of pair j + 1 during j.
+    const float *cosq = A.cosb + qoff, *sinq = A.sinb + qoff;
+    // The loads are unconditional (indices clamped to the last pair, whose
+    // window is then staged twice into the idle buffer) so that no branch
+    // merges force early waits on them.
+    float cn = cosq[jlo], sn = sinq[jlo];
+    float c2n = cosq[jlo + A.h], s2n_ = sinq[jlo + A.h];
+    unsigned zmn = A.zmask[0];
+    const int jl1 = min(jlo + 1, jhi - 1);
+    float cnn = cosq[jl1], snn = sinq[jl1];
     int s1n, s2n;
-    win_of(jlo, s1n, s2n);
+    win_of(cn, sn, s1n, s2n);
     fetch(jlo, s1n, s2n);
     commit(lds);
     __syncthreads();
     int cur = 0;
     for (int j = jlo; j < jhi; j++) {
         const int s1 = s1n, s2 = s2n;
-        if (j + 1 < jhi) {
-            win_of(j + 1, s1n, s2n);
-            fetch(j + 1, s1n, s2n);
-        }
+        const float c = cn, s = sn, c2 = c2n, s2v = s2n_;
+        const unsigned zm = __builtin_amdgcn_readfirstlane(zmn);
+        const int j1 = min(j + 1, jhi - 1), j2 = min(j + 2, jhi - 1);
+        win_of(cnn, snn, s1n, s2n);
+        fetch(j1, s1n, s2n);
+        cn = cnn;
+        sn = snn;
+        c2n = cosq[j1 + A.h];
+        s2n_ = sinq[j1 + A.h];
+        zmn = A.zmask[j1 - jlo];
+        cnn = cosq[j2];
+        snn = sinq[j2];
         const float *buf = lds + cur * stage;
         // z-independent terms of the pair (Backprojector.cu:53-62)
-        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
-        const float c2 = A.cosb[qoff + j + A.h], s2v = A.sinb[qoff + j + A.h];
         const float phat = -xc * c - yc * s;
         const float theta = asinf(phat * A.inv_dso);
         const float delta = theta * A.dso_ds;
@@ -275,8 +319,8 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         // clamp only keeps LDS addressing inside the stage buffer
         const int l1 = min(max(int(fp1) + 1 - s1, 0), W - 2);
         const int l2 = min(max(int(fp2) + 1 - s2, 0), W - 2);
-        const float *b1 = buf + l1 * rp;
-        const float *b2 = buf + view_elems + l2 * rp;
+        const float *b1 = buf + __mul24(l1, lp);
+        const float *b2 = buf + lds_view + __mul24(l2, lp);
         const float root = sqrtf(A.dso2 - phat * phat);
         const float lhat = root + xc * s - yc * c;
         const float lhat2 = root + xc * s2v - yc * c2;
@@ -291,12 +335,15 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const float lo_r = float(kRowPad) - 1.0f, hi_r = A.nt_f + float(kRowPad);
 #pragma unroll
         for (int z = 0; z < 32; z++) {
-            if (am[z] > j && am[z] <= j + A.h) {  // pair inside this slice's window
+            if (zm & (1u << z)) {  // pair inside this slice's window
                 const float zc = (float(z) - A.wz) * A.dz;
                 const float qh = fmaf(zc, k1, o1);
                 const float qh2 = fmaf(zc, k2, o2);
                 const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
-                const float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
+                float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
+                // keeps wb computed on every lane, so the selects below stay
+                // selects instead of divergent branches (emits no instruction)
+                asm volatile("" : "+v"(wb));
                 // conjugate weights (w1, 1 - w1); both rays off the detector: 1/2 each
                 const float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
                 const float r1 = __builtin_amdgcn_fmed3f(qh + row0, lo_r, hi_r);
@@ -305,8 +352,8 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
                 const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
                 const float *p1 = b1 + i1;
                 const float *p2 = b2 + i2;
-                const float a00 = p1[0], a01 = p1[1], a10 = p1[rp], a11 = p1[rp + 1];
-                const float c00 = p2[0], c01 = p2[1], c10 = p2[rp], c11 = p2[rp + 1];
+                const float a00 = p1[0], a01 = p1[1], a10 = p1[lp], a11 = p1[lp + 1];
+                const float c00 = p2[0], c01 = p2[1], c10 = p2[lp], c11 = p2[lp + 1];
                 const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
                 const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
                 const float t1 = fmaf(ar1, ha - la, la);
@@ -314,7 +361,7 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
                 acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
             }
         }
-        if (j + 1 < jhi) commit(lds + (cur ^ 1) * stage);
+        commit(lds + (cur ^ 1) * stage);
         __syncthreads();
         cur ^= 1;
     }
@@ -425,21 +472,22 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     constexpr int CE = Chunk<T>::kElems;
+    a.lp = a.rp + 1;
     const int chunks = 2 * a.W * a.rp / CE;
-    const size_t shm = size_t(2) * 2 * a.W * a.rp * sizeof(float);
+    const size_t shm = size_t(2) * 2 * a.W * a.lp * sizeof(float);
     if (shm > 160 * 1024)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
     const int npre = (chunks + 255) / 256;
     if (npre < 1 || npre > 8)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window too large");
-    // row strides of the 16-, 64- and 96-row detectors get immediate LDS offsets
+    // LDS pitches of the 16-, 64- and 96-row detectors get immediate offsets
     using Kern = void (*)(FoldBpArgs);
     static const Kern table[4][8] = {
-#define LCT_ROW(RP)                                                                              \
-    {bp_fold_kernel<T, 1, RP>, bp_fold_kernel<T, 2, RP>, bp_fold_kernel<T, 3, RP>,               \
-     bp_fold_kernel<T, 4, RP>, bp_fold_kernel<T, 5, RP>, bp_fold_kernel<T, 6, RP>,               \
-     bp_fold_kernel<T, 7, RP>, bp_fold_kernel<T, 8, RP>}
-        LCT_ROW(0), LCT_ROW(24), LCT_ROW(72), LCT_ROW(104)
+#define LCT_ROW(LP)                                                                              \
+    {bp_fold_kernel<T, 1, LP>, bp_fold_kernel<T, 2, LP>, bp_fold_kernel<T, 3, LP>,               \
+     bp_fold_kernel<T, 4, LP>, bp_fold_kernel<T, 5, LP>, bp_fold_kernel<T, 6, LP>,               \
+     bp_fold_kernel<T, 7, LP>, bp_fold_kernel<T, 8, LP>}
+        LCT_ROW(0), LCT_ROW(25), LCT_ROW(73), LCT_ROW(105)
 #undef LCT_ROW
     };
     const int row = a.rp == 24 ? 1 : a.rp == 72 ? 2 : a.rp == 104 ? 3 : 0;
@@ -461,9 +509,12 @@ static int tile_half_width(const lct_ct_geom &g, const lct_img_geom &ig)
     double u = std::sqrt(rx * rx + ry * ry) / g.dso;
     u = std::min(u, 0.999);
     const double slope = 1.0 / std::sqrt(1.0 - u * u);
-    // |dx cos b + dy sin b| <= sqrt(dx^2 + dy^2) for the column offsets of a tile
+    // |dx cos b + dy sin b| <= sqrt(dx^2 + dy^2) for the column offsets of a tile.
+    // Taps of a column at channel p use channels floor(p) and floor(p) + 1, and
+    // |p - c| <= E for the tile centre c, so H >= ceil(E) covers them; one
+    // more channel absorbs FP32 rounding of the two asin evaluations.
     const double half_extent = 0.5 * (kTile - 1) * std::sqrt(double(ig.dx) * ig.dx + double(ig.dy) * ig.dy);
-    return int(std::ceil(half_extent * slope / g.ds)) + 2;
+    return int(std::ceil(half_extent * slope / g.ds)) + 1;
 }
 
 struct Resident {
@@ -618,12 +669,23 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         tab[34 * b + 1] = b * LCT_BLOCK_SLICES;
         for (int z = 0; z < 32; z++) tab[34 * b + 2 + z] = amid[z];
     }
-    const size_t tab_bytes = cs_host.size() * 4 + tab.size() * 4;
+    // slices whose window holds pair j (a_middle is the same in every block)
+    std::vector<unsigned> zmask(size_t(amid[31] - amid[0] + h));
+    for (size_t jj = 0; jj < zmask.size(); jj++) {
+        const int j = amid[0] - h + int(jj);
+        unsigned m = 0;
+        for (int z = 0; z < 32; z++)
+            if (amid[z] - h <= j && j < amid[z]) m |= 1u << z;
+        zmask[jj] = m;
+    }
+    const size_t tab_bytes = cs_host.size() * 4 + tab.size() * 4 + zmask.size() * 4;
     LCT_CHECK(scratch(ctx, kScratchTables, tab_bytes, &d_tab));
     float *d_cs = static_cast<float *>(d_tab);
     int *d_st = reinterpret_cast<int *>(d_cs + cs_host.size());
+    unsigned *d_zm = reinterpret_cast<unsigned *>(d_st + tab.size());
     LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
     LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
+    LCT_CHECK(upload(ctx, d_zm, zmask.data(), zmask.size() * 4));
 
     FoldBpArgs a{};
     a.q = d_q;
@@ -635,6 +697,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.cosb = d_cs;
     a.sinb = d_cs + n_cs;
     a.slab_tab = d_st;
+    a.zmask = d_zm;
     a.vol = d_vol;
     a.n_tiles_x = (ig.nx + kTile - 1) / kTile;
     a.n_tiles_y = (ig.ny + kTile - 1) / kTile;
