@@ -142,6 +142,7 @@ struct FoldBpArgs {
     int rp, cp, W, H;    // channel window of W = 2H + 2 slots
     int lp;              // LDS row pitch of a staged channel (rp + 1: odd, so the
                          // channels that neighbouring lanes read fall in different banks)
+    int asin_poly;       // every voxel column has |p_hat| / dso <= 0.5: asin_poly is valid
     const float *cosb, *sinb;  // indexed by Q view
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
     const unsigned *zmask;     // per pair j - (a_middle[0] - h): bit z set when
@@ -182,6 +183,22 @@ __device__ __forceinline__ void chunk_to_lds_odd(float *dst, const uint4 &v)
     }
 }
 
+// asin(u) for |u| <= 0.5 as u + u^3 P(u^2), P of degree 5: coefficients from
+// a weighted least-squares fit of (asin(u) - u) / u^3 on Chebyshev nodes in
+// u^2, reweighted toward equal relative error.  Evaluated in FP32 it stays
+// within 0.6 ulp of the exact asin over the whole interval.
+__device__ __forceinline__ float asin_poly(float u)
+{
+    const float t = u * u;
+    float p = 0.0375651829f;
+    p = fmaf(p, t, 0.0144026186f);
+    p = fmaf(p, t, 0.0318176076f);
+    p = fmaf(p, t, 0.0445157215f);
+    p = fmaf(p, t, 0.0750051066f);
+    p = fmaf(p, t, 0.166666597f);
+    return fmaf(u * t, p, u);
+}
+
 // First channel slot of the W-slot window of a view, from the channel
 // coordinate c of the tile centre (window covers every tap of the tile).
 __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
@@ -206,7 +223,11 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     const int view_elems = W * rp;          // one view's window in Q
     const int lds_view = W * lp;            // and in LDS
     const int stage = 2 * lds_view;         // views j and j+h
-    const int n_chunks = 2 * view_elems / CE;
+    // 16-byte chunks: view j's at chunk indices [0, n_vc), view j+h's at
+    // [pad, pad + n_vc) with pad a multiple of 64, so each wave's chunks of a
+    // slot belong to one view and share a scalar base address
+    const int n_vc = view_elems / CE;
+    const int pad = (n_vc + 63) & ~63;
 
     // XCD-aware order: consecutive workgroups are dealt round-robin to the 8
     // XCDs; give each XCD a contiguous range of tiles so the views it stages
@@ -243,73 +264,88 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         s1 = window_start(c1, A.H, A.cp, W);
         s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, W);
     };
+    // Chunk slot i of this thread: chunk index c = tid + 256 i, view
+    // (c >= pad, the same on every lane of a wave), element offset in the
+    // view's window and LDS offset, all fixed for the whole loop (a chunk
+    // never straddles a channel: rp is a multiple of 8).
+    int src_off[NPRE], dst[NPRE];
+    bool live[NPRE];
+#pragma unroll
+    for (int i = 0; i < NPRE; i++) {
+        const int c = int(threadIdx.x) + i * 256;
+        const int v = c >= pad ? 1 : 0;
+        const int m = c - v * pad;
+        live[i] = m < n_vc;
+        const int e = (live[i] ? m : 0) * CE;
+        src_off[i] = e;
+        dst[i] = v * lds_view + (e / rp) * lp + e % rp;
+    }
     auto fetch = [&](int j, int s1, int s2) {
         const T *va = Q + size_t(qoff + j) * A.view_stride + size_t(s1) * rp;
         const T *vb = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(s2) * rp;
 #pragma unroll
         for (int i = 0; i < NPRE; i++) {
-            const int c = int(threadIdx.x) + i * 256;
-            if (c < n_chunks) {
-                const int e = c * CE;
-                const T *src = e < view_elems ? va + e : vb + (e - view_elems);
-                pre[i] = *reinterpret_cast<const uint4 *>(src);
-            }
+            const bool vbw = __builtin_amdgcn_readfirstlane(int(threadIdx.x) + i * 256) >= pad;
+            const T *base = vbw ? vb : va;
+            if (live[i]) pre[i] = *reinterpret_cast<const uint4 *>(base + unsigned(src_off[i]));
         }
     };
-    // LDS offset of each chunk this thread stages (a chunk never straddles a
-    // channel: rp is a multiple of 8)
-    int dst[NPRE];
-#pragma unroll
-    for (int i = 0; i < NPRE; i++) {
-        const int e = (int(threadIdx.x) + i * 256) * CE;
-        const int v = e < view_elems ? 0 : 1;
-        const int ev = e - v * view_elems;
-        dst[i] = v * lds_view + (ev / rp) * lp + ev % rp;
-    }
     auto commit = [&](float *buf) {
 #pragma unroll
-        for (int i = 0; i < NPRE; i++) {
-            const int c = int(threadIdx.x) + i * 256;
-            if (c < n_chunks) chunk_to_lds_odd<T>(buf + dst[i], pre[i]);
-        }
+        for (int i = 0; i < NPRE; i++)
+            if (live[i]) chunk_to_lds_odd<T>(buf + dst[i], pre[i]);
     };
 
-    // Per-pair scalars are loaded ahead so no pair waits on them: cos/sin of
-    // pair j + 1's first view during pair j - 1 (pair j stages j + 1's window),
-    // the conjugate view's cos/sin and the slice mask of pair j + 1 during j.
+    // Per-pair scalars of 64 consecutive pairs [b0, b0 + 64), one pair per
+    // lane (cos/sin of both views, slice mask, both window starts); pair j
+    // reads them from lane j - b0.  Refilled every 64 pairs, so the uniform
+    // window asin and the table loads are not repeated on every lane.
+    const int lane = int(threadIdx.x) & 63;
     const float *cosq = A.cosb + qoff, *sinq = A.sinb + qoff;
-    // The loads are unconditional (indices clamped to the last pair, whose
-    // window is then staged twice into the idle buffer) so that no branch
-    // merges force early waits on them.
-    float cn = cosq[jlo], sn = sinq[jlo];
-    float c2n = cosq[jlo + A.h], s2n_ = sinq[jlo + A.h];
-    unsigned zmn = A.zmask[0];
-    const int jl1 = min(jlo + 1, jhi - 1);
-    float cnn = cosq[jl1], snn = sinq[jl1];
-    int s1n, s2n;
-    win_of(cn, sn, s1n, s2n);
-    fetch(jlo, s1n, s2n);
+    float bc, bs, bc2, bs2;
+    int bzm, bwin;
+    auto batch = [&](int b) {
+        const int jj = min(b + lane, jhi - 1);
+        bc = cosq[jj];
+        bs = sinq[jj];
+        bc2 = cosq[jj + A.h];
+        bs2 = sinq[jj + A.h];
+        bzm = int(A.zmask[jj - jlo]);
+        int w1, w2;
+        win_of(bc, bs, w1, w2);
+        bwin = w1 | (w2 << 16);
+    };
+    auto lanef = [](float v, int l) {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    };
+    int b0 = jlo;
+    batch(b0);
+    {
+        const int w = __builtin_amdgcn_readlane(bwin, 0);
+        fetch(jlo, w & 0xffff, w >> 16);
+    }
     commit(lds);
     __syncthreads();
     int cur = 0;
     for (int j = jlo; j < jhi; j++) {
-        const int s1 = s1n, s2 = s2n;
-        const float c = cn, s = sn, c2 = c2n, s2v = s2n_;
-        const unsigned zm = __builtin_amdgcn_readfirstlane(zmn);
-        const int j1 = min(j + 1, jhi - 1), j2 = min(j + 2, jhi - 1);
-        win_of(cnn, snn, s1n, s2n);
-        fetch(j1, s1n, s2n);
-        cn = cnn;
-        sn = snn;
-        c2n = cosq[j1 + A.h];
-        s2n_ = sinq[j1 + A.h];
-        zmn = A.zmask[j1 - jlo];
-        cnn = cosq[j2];
-        snn = sinq[j2];
+        const int l = j - b0;
+        const float c = lanef(bc, l), s = lanef(bs, l);
+        const float c2 = lanef(bc2, l), s2v = lanef(bs2, l);
+        const unsigned zm = unsigned(__builtin_amdgcn_readlane(bzm, l));
+        const int wj = __builtin_amdgcn_readlane(bwin, l);
+        const int s1 = wj & 0xffff, s2 = wj >> 16;
+        if (l == 63) {
+            b0 += 64;
+            batch(b0);
+        }
+        // stage pair j + 1 (the last pair is staged twice, into the idle buffer)
+        const int wn = __builtin_amdgcn_readlane(bwin, j + 1 - b0);
+        fetch(min(j + 1, jhi - 1), wn & 0xffff, wn >> 16);
         const float *buf = lds + cur * stage;
         // z-independent terms of the pair (Backprojector.cu:53-62)
         const float phat = -xc * c - yc * s;
-        const float theta = asinf(phat * A.inv_dso);
+        const float u = phat * A.inv_dso;
+        const float theta = A.asin_poly ? asin_poly(u) : asinf(u);
         const float delta = theta * A.dso_ds;
         const float p1 = fminf(fmaxf(A.ws + delta, -1.0f), A.ns_f);
         const float p2 = fminf(fmaxf(A.ws - delta, -1.0f), A.ns_f);
@@ -321,7 +357,7 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const int l2 = min(max(int(fp2) + 1 - s2, 0), W - 2);
         const float *b1 = buf + __mul24(l1, lp);
         const float *b2 = buf + lds_view + __mul24(l2, lp);
-        const float root = sqrtf(A.dso2 - phat * phat);
+        const float root = __builtin_amdgcn_sqrtf(A.dso2 - phat * phat);
         const float lhat = root + xc * s - yc * c;
         const float lhat2 = root + xc * s2v - yc * c2;
         const float tz = theta * A.inv_orbit;
@@ -473,7 +509,9 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     constexpr int CE = Chunk<T>::kElems;
     a.lp = a.rp + 1;
-    const int chunks = 2 * a.W * a.rp / CE;
+    // chunk slots: view j's chunks, padded to a multiple of 64, then view j+h's
+    const int n_vc = a.W * a.rp / CE;
+    const int chunks = ((n_vc + 63) & ~63) + n_vc;
     const size_t shm = size_t(2) * 2 * a.W * a.lp * sizeof(float);
     if (shm > 160 * 1024)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
@@ -500,14 +538,20 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     return LCT_OK;
 }
 
+// Bound on |p_hat| / dso = sin(theta) over the volume's voxel columns (one
+// voxel of margin).
+static double column_u_max(const lct_ct_geom &g, const lct_img_geom &ig)
+{
+    const double rx = (std::max(std::fabs(0.0 - ig.wx), std::fabs(ig.nx - 1.0 - ig.wx)) + 1.0) * ig.dx;
+    const double ry = (std::max(std::fabs(0.0 - ig.wy), std::fabs(ig.ny - 1.0 - ig.wy)) + 1.0) * ig.dy;
+    return std::sqrt(rx * rx + ry * ry) / g.dso;
+}
+
 // Half-width (in channels) of the channel footprint of a 16x16 tile, with
 // the largest in-plane radius of the volume bounding the asin slope.
 static int tile_half_width(const lct_ct_geom &g, const lct_img_geom &ig)
 {
-    const double rx = (std::max(std::fabs(0.0 - ig.wx), std::fabs(ig.nx - 1.0 - ig.wx)) + 1.0) * ig.dx;
-    const double ry = (std::max(std::fabs(0.0 - ig.wy), std::fabs(ig.ny - 1.0 - ig.wy)) + 1.0) * ig.dy;
-    double u = std::sqrt(rx * rx + ry * ry) / g.dso;
-    u = std::min(u, 0.999);
+    const double u = std::min(column_u_max(g, ig), 0.999);
     const double slope = 1.0 / std::sqrt(1.0 - u * u);
     // |dx cos b + dy sin b| <= sqrt(dx^2 + dy^2) for the column offsets of a tile.
     // Taps of a column at channel p use channels floor(p) and floor(p) + 1, and
@@ -693,6 +737,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.rp = rp;
     a.cp = cp;
     a.H = tile_half_width(g, ig);
+    a.asin_poly = column_u_max(g, ig) <= 0.5 ? 1 : 0;
     a.W = std::min(2 * a.H + 2, cp);
     a.cosb = d_cs;
     a.sinb = d_cs + n_cs;
