@@ -190,6 +190,7 @@ def main():
     achieved = alg_bytes / bp_s
     gups = updates / (head["stage_ms"]["backproject_ms"] * 1e-3) / 1e9
     tag = f"bp_fold_{storage}_{w.name}"
+    traffic = load_traffic(tag)
 
     out = {
         "metric": METRIC,
@@ -216,10 +217,13 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": load_traffic(tag),
+            "traffic": traffic,
+            # measured HBM bytes (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)
+            # over this run's launch time: the share of HBM bandwidth really used
+            "traffic_frac_of_peak": (round(traffic / bp_s / HBM_PEAK, 4) if traffic else None),
             "definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update "
                           f"+ 4 B per voxel written, per launch; {updates / launches:.4g} updates "
-                          f"per launch",
+                          f"per launch; the taps are read from LDS, so frac > 1 is possible",
         },
     }
     if variants:

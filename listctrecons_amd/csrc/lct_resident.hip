@@ -142,7 +142,7 @@ struct FoldBpArgs {
     int rp, cp, W, H;    // channel window of W = 2H + 2 slots
     int lp;              // LDS row pitch of a staged channel (rp + 1: odd, so the
                          // channels that neighbouring lanes read fall in different banks)
-    int asin_poly;       // every voxel column has |p_hat| / dso <= 0.5: asin_poly is valid
+    int asin_poly;       // every voxel column has |p_hat| / dso <= 0.7: asin_poly is valid
     const float *cosb, *sinb;  // indexed by Q view
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
     const unsigned *zmask;     // per pair j - (a_middle[0] - h): bit z set when
@@ -183,19 +183,20 @@ __device__ __forceinline__ void chunk_to_lds_odd(float *dst, const uint4 &v)
     }
 }
 
-// asin(u) for |u| <= 0.5 as u + u^3 P(u^2), P of degree 5: coefficients from
+// asin(u) for |u| <= 0.7 as u + u^3 P(u^2), P of degree 6: coefficients from
 // a weighted least-squares fit of (asin(u) - u) / u^3 on Chebyshev nodes in
 // u^2, reweighted toward equal relative error.  Evaluated in FP32 it stays
-// within 0.6 ulp of the exact asin over the whole interval.
+// within 0.83 ulp of the exact asin over the whole interval.
 __device__ __forceinline__ float asin_poly(float u)
 {
     const float t = u * u;
-    float p = 0.0375651829f;
-    p = fmaf(p, t, 0.0144026186f);
-    p = fmaf(p, t, 0.0318176076f);
-    p = fmaf(p, t, 0.0445157215f);
-    p = fmaf(p, t, 0.0750051066f);
-    p = fmaf(p, t, 0.166666597f);
+    float p = 0.106134832f;
+    p = fmaf(p, t, -0.0844023898f);
+    p = fmaf(p, t, 0.0719100833f);
+    p = fmaf(p, t, 0.0179117583f);
+    p = fmaf(p, t, 0.0462728813f);
+    p = fmaf(p, t, 0.0749005452f);
+    p = fmaf(p, t, 0.166668698f);
     return fmaf(u * t, p, u);
 }
 
@@ -737,7 +738,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.rp = rp;
     a.cp = cp;
     a.H = tile_half_width(g, ig);
-    a.asin_poly = column_u_max(g, ig) <= 0.5 ? 1 : 0;
+    a.asin_poly = column_u_max(g, ig) <= 0.7 ? 1 : 0;
     a.W = std::min(2 * a.H + 2, cp);
     a.cosb = d_cs;
     a.sinb = d_cs + n_cs;

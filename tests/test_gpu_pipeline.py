@@ -72,6 +72,10 @@ VARIANTS = [
     ("tiny", {}, {"rebin_kind": 0}),
     ("tiny", {}, {"rebin_kind": 2, "weight_kind": 1, "bp_kind": 1}),
     ("tiny", {"n_proj_turn": 97, "readings": 1200}, {}),  # odd npt: no pair folding
+    # volume corners at |p_hat| / dso ~ 0.6 (the backprojector's asin
+    # polynomial) and ~ 0.73 (beyond its range: the library asin)
+    ("tiny", {"dx": 14.0}, {}),
+    ("tiny", {"dx": 17.0}, {}),
 ]
 
 
