@@ -124,6 +124,13 @@ int lct_plan_helical(const lct_prm *prm, const float *h_angles, const float *h_c
  * (array_search(blockStart)), for mapping slices to table positions. */
 int lct_block_info(const lct_plan *plan, const float *h_couch, int block, int *proj_start,
                    int *proj_end, float *block_start, float *block_end, int *ref_view);
+/* Projection-file views [*view_begin, *view_end) that blocks
+ * [block_begin, block_end) read: the union of their BlockInfo windows
+ * (Src/DataStructs.cpp:77-108) with DataOffset applied (RawDataReader,
+ * Src/DataReader.cpp:14).  Host only.  What a caller streaming a scan slab
+ * by slab uploads for lct_reconstruct_slabs. */
+int lct_slab_views(const lct_plan *plan, const float *h_couch, int block_begin, int block_end,
+                   long *view_begin, long *view_end);
 
 /* -------------------------------------------------------- stage kernels
  * Per-block semantics of the reference stages: one call processes one
@@ -200,6 +207,18 @@ typedef struct {
 int lct_reconstruct(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
                     const float *h_angles, const float *h_couch, const float *d_fan,
                     long file_views, float *d_vol);
+/* lct_reconstruct for blocks [block_begin, block_end) of the plan only (the
+ * reference's per-block loop, Src/Reconstructor.cpp:33-48, over a range).
+ * d_fan holds projection-file views [fan_view0, fan_view0 + fan_views),
+ * which must cover lct_slab_views of the range (LCT_ERR_RANGE otherwise);
+ * d_vol receives (block_end - block_begin) * 32 slices, its slice 0 being
+ * slice 32 * block_begin of the plan.  Every slice equals the same slice of
+ * lct_reconstruct: a long scan can be streamed through HBM slab by slab and
+ * checkpointed between slabs. */
+int lct_reconstruct_slabs(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
+                          const float *h_angles, const float *h_couch, const float *d_fan,
+                          long fan_view0, long fan_views, int block_begin, int block_end,
+                          float *d_vol);
 int lct_get_timings(lct_ctx *ctx, lct_timings *out);
 /* Enables HIP-event timing of every stage launch (off by default). */
 int lct_set_profiling(lct_ctx *ctx, int enable);

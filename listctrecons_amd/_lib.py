@@ -122,6 +122,7 @@ SIGNATURES = {
     "lct_memset_zero": (C.c_int, [_vp, _vp, C.c_size_t]),
     "lct_plan_helical": (C.c_int, [_P(Prm), _fp, _fp, _P(Plan)]),
     "lct_block_info": (C.c_int, [_P(Plan), _fp, C.c_int, _ip, _ip, _fp, _fp, _ip]),
+    "lct_slab_views": (C.c_int, [_P(Plan), _fp, C.c_int, C.c_int, _P(C.c_long), _P(C.c_long)]),
     "lct_ramp_taps": (C.c_int, [C.c_int, C.c_int, C.c_float, _fp]),
     "lct_rebin": (C.c_int, [_vp, C.c_int, C.c_int, _P(CTGeom), _vp, _vp]),
     "lct_weighting": (C.c_int, [_vp, C.c_int, _P(CTGeom), _vp]),
@@ -130,6 +131,8 @@ SIGNATURES = {
     "lct_filter_to_bp_layout": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _vp,
                                           C.c_long, _vp]),
     "lct_reconstruct": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long, _vp]),
+    "lct_reconstruct_slabs": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long,
+                                        C.c_long, C.c_int, C.c_int, _vp]),
     "lct_get_timings": (C.c_int, [_vp, _P(Timings)]),
     "lct_set_profiling": (C.c_int, [_vp, C.c_int]),
     "lct_project_ellipsoids": (C.c_int, [_vp, _P(CTGeom), _fp, _fp, C.c_long, C.c_long, _dp,

@@ -7,11 +7,16 @@
 #include <cctype>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
 
+#include <sys/types.h>
+#include <unistd.h>
+
 #include "Backprojector.h"
+#include "Common.h"
 #include "DataReader.h"
 #include "DataStructs.h"
 #include "Filter.h"
@@ -54,54 +59,149 @@ void FDKReconstructor::reconstruct(const ReconData *mr)
     }
 }
 
+namespace {
+
+[[noreturn]] void fail(const std::string &what)
+{
+    std::cout << "[ERROR] " << what << std::endl;
+    std::exit(1);
+}
+
+// Identity of a streamed reconstruction: a checkpoint is only resumed by a
+// run with the same plan shape, slab size and options.
+std::string checkpoint_key(const lct_plan &p, const lct_recon_opts &o, int per)
+{
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "plan %d %d %d %d %d slab %d opts %d %d %d %d %d %d %d %d", p.n_blocks,
+                  p.n_slices_recon, p.ig.nx, p.ig.ny, p.ctg.n_angle_add, per, o.pipeline,
+                  o.rebin_kind, o.weight_kind, o.window, o.filter_algo, o.storage, o.bp_kind,
+                  o.interp);
+    return buf;
+}
+
+// Blocks already done by an earlier run (0 when there is no checkpoint);
+// exits when the checkpoint belongs to another reconstruction.
+int read_checkpoint(const std::string &path, const std::string &key)
+{
+    std::ifstream f(path.c_str());
+    if (!f.is_open()) return 0;
+    std::string magic, k, done;
+    std::getline(f, magic);
+    std::getline(f, k);
+    std::getline(f, done);
+    int n = -1;
+    if (magic != "LISTCTRecons checkpoint 1" || k != key ||
+        std::sscanf(done.c_str(), "done %d", &n) != 1 || n < 0)
+        fail("checkpoint " + path + " belongs to another reconstruction; remove it to start over");
+    return n;
+}
+
+void write_checkpoint(const std::string &path, const std::string &key, int done)
+{
+    const std::string tmp = path + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "w");
+    if (!f || std::fprintf(f, "LISTCTRecons checkpoint 1\n%s\ndone %d\n", key.c_str(), done) < 0 ||
+        std::fflush(f) != 0 || fsync(fileno(f)) != 0 || std::fclose(f) != 0 ||
+        std::rename(tmp.c_str(), path.c_str()) != 0)
+        fail("could not write checkpoint " + path);
+}
+
+bool has(const std::string &s, const char *t) { return s.find(t) != std::string::npos; }
+
+}  // namespace
+
 void FDKResidentReconstructor::reconstruct(const ReconData *mr)
 {
     std::cout << "Start reconstruction (HBM-resident views)..." << std::endl;
-    lct_plan plan = *mr->plan_();
+    const lct_plan &plan = *mr->plan_();
     const float *couch = mr->data_cpu()->table_positions();
     const float *angles = mr->data_cpu()->tube_angles();
-    // file views [lo, hi) cover every block's window
-    long lo = 1L << 40, hi = 0;
-    for (int b = 0; b < plan.n_blocks; b++) {
-        int s = 0;
-        lct_block_info(&plan, couch, b, &s, nullptr, nullptr, nullptr, nullptr);
-        lo = std::min(lo, long(s) + plan.data_offset);
-        hi = std::max(hi, long(s) + plan.data_offset + plan.ctg.n_angle_add);
-    }
-    if (lo < 0) {
-        std::cout << "[ERROR] projection window starts before the data file" << std::endl;
-        std::exit(1);
-    }
     const size_t view = size_t(plan.ctg.ns) * plan.ctg.nt;
-    const size_t n = view * size_t(hi - lo);
-    std::vector<float> host(n);
-    const std::string path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
-    FILE *f = std::fopen(path.c_str(), "rb");
-    if (!f || std::fseek(f, long(lo * long(view) * long(sizeof(float))), SEEK_SET) != 0 ||
-        std::fread(host.data(), sizeof(float), n, f) != n) {
-        std::cout << "[ERROR] could not read " << n << " values from " << path << std::endl;
-        std::exit(1);
-    }
-    std::fclose(f);
-    GPUMemoryCtrl<float> fan, vol;
-    const size_t n_vol = size_t(plan.ig.nx) * plan.ig.ny * plan.n_slices_recon;
-    if (!fan.allocate_memory(n) || !fan.copy_from_CPU(host.data(), n) || !vol.allocate_memory(n_vol)) {
-        std::cout << "[ERROR] " << lct_last_error() << std::endl;
-        std::exit(1);
-    }
-    plan.data_offset -= int(lo);  // the device buffer starts at file view `lo`
-    if (lct_reconstruct(lct_default_ctx(), &plan, &opts, angles, couch, fan.get_pointer(),
-                        hi - lo, vol.get_pointer()) != LCT_OK ||
-        lct_synchronize(lct_default_ctx()) != LCT_OK) {
-        std::cout << "[ERROR] " << lct_last_error() << std::endl;
-        std::exit(1);
-    }
-    vol.copy_to_CPU(mr->data_cpu()->image_(), n_vol);
-    std::cout << "Reconstruction finished (" << plan.n_blocks << " blocks)." << std::endl;
-}
+    const size_t slice = size_t(plan.ig.nx) * plan.ig.ny;
+    const int per = slab_blocks > 0 ? std::min(slab_blocks, plan.n_blocks) : plan.n_blocks;
+    float *image = mr->data_cpu()->image_();
 
-namespace {
-bool has(const std::string &s, const char *t) { return s.find(t) != std::string::npos; }
+    // resume point
+    const std::string out_path = mr->fi_()->output_dir() + mr->fi_()->output_name();
+    const std::string ck_path = out_path + ".ckpt";
+    const std::string key = checkpoint_key(plan, opts, per);
+    int start = 0;
+    FILE *out = nullptr;
+    if (checkpoint) {
+        start = read_checkpoint(ck_path, key);
+        if (start > plan.n_blocks || start % per != 0)
+            fail("checkpoint " + ck_path + " records " + std::to_string(start) + " blocks");
+        out = std::fopen(out_path.c_str(), start > 0 ? "r+b" : "w+b");
+        if (!out) fail("could not open " + out_path);
+        if (start > 0) {
+            const size_t n = size_t(start) * BLOCK_SLICES * slice;
+            if (std::fread(image, sizeof(float), n, out) != n)
+                fail(out_path + " holds fewer slices than checkpoint " + ck_path + " records");
+            std::cout << "Resuming after block " << start - 1 << " (" << ck_path << ")." << std::endl;
+        }
+    }
+
+    // device buffers sized for the largest slab
+    long max_views = 0;
+    for (int b0 = start; b0 < plan.n_blocks; b0 += per) {
+        long v0 = 0, v1 = 0;
+        if (lct_slab_views(&plan, couch, b0, std::min(b0 + per, plan.n_blocks), &v0, &v1) != LCT_OK)
+            fail(lct_last_error());
+        if (v0 < 0) fail("projection window starts before the data file");
+        max_views = std::max(max_views, v1 - v0);
+    }
+    GPUMemoryCtrl<float> fan, vol;
+    if (start < plan.n_blocks &&
+        (!fan.allocate_memory(view * size_t(max_views)) ||
+         !vol.allocate_memory(size_t(per) * BLOCK_SLICES * slice)))
+        fail(lct_last_error());
+
+    const std::string in_path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
+    std::vector<float> host;
+    int run = 0;
+    for (int b0 = start; b0 < plan.n_blocks; b0 += per) {
+        const int b1 = std::min(b0 + per, plan.n_blocks);
+        long v0 = 0, v1 = 0;
+        lct_slab_views(&plan, couch, b0, b1, &v0, &v1);
+        const size_t n = view * size_t(v1 - v0);
+        host.resize(n);
+        FILE *f = std::fopen(in_path.c_str(), "rb");
+        if (!f || fseeko(f, off_t(v0) * off_t(view) * off_t(sizeof(float)), SEEK_SET) != 0 ||
+            std::fread(host.data(), sizeof(float), n, f) != n) {
+            if (f) std::fclose(f);
+            fail("could not read views " + std::to_string(v0) + " to " + std::to_string(v1) +
+                 " from " + in_path);
+        }
+        std::fclose(f);
+        const size_t n_out = size_t(b1 - b0) * BLOCK_SLICES * slice;
+        float *dst = image + size_t(b0) * BLOCK_SLICES * slice;
+        if (!fan.copy_from_CPU(host.data(), n) ||
+            lct_reconstruct_slabs(lct_default_ctx(), &plan, &opts, angles, couch,
+                                  fan.get_pointer(), v0, v1 - v0, b0, b1,
+                                  vol.get_pointer()) != LCT_OK ||
+            lct_synchronize(lct_default_ctx()) != LCT_OK || !vol.copy_to_CPU(dst, n_out))
+            fail(lct_last_error());
+        if (checkpoint) {
+            if (fseeko(out, off_t(b0) * BLOCK_SLICES * off_t(slice) * off_t(sizeof(float)),
+                       SEEK_SET) != 0 ||
+                std::fwrite(dst, sizeof(float), n_out, out) != n_out || std::fflush(out) != 0 ||
+                fsync(fileno(out)) != 0)
+                fail("could not write slab to " + out_path);
+            write_checkpoint(ck_path, key, b1);
+        }
+        if (per < plan.n_blocks)
+            std::cout << "Blocks " << b0 << " to " << b1 - 1 << " finished." << std::endl;
+        if (stop_after_slabs > 0 && ++run == stop_after_slabs && b1 < plan.n_blocks) {
+            std::cout << "Stopping after " << run << " slab(s)." << std::endl;
+            std::exit(3);
+        }
+    }
+    if (checkpoint) {
+        std::fclose(out);
+        std::remove(ck_path.c_str());
+    }
+    std::cout << "Reconstruction finished (" << plan.n_blocks << " blocks)." << std::endl;
 }
 
 // The reference tests `str.find("fdk")`, which is true for every string that
@@ -114,6 +214,10 @@ bool has(const std::string &s, const char *t) { return s.find(t) != std::string:
 //   "resident"     FDKResidentReconstructor, FP32 storage
 //   "f16"/"bf16"   FDKResidentReconstructor, f16 / bf16 filtered projections
 //                  with the BF16x3 Toeplitz MFMA ramp filter
+//   "slab<N>"      FDKResidentReconstructor streaming slabs of N blocks
+//                  ("slab" alone: N = 1)
+//   "ckpt"         ... writing each finished slab and a checkpoint, resuming
+//                  an interrupted run (implies slabs of 1 block unless given)
 Reconstructor *ReconstrctorAbstractFactory::createReconstructor(const std::string &str)
 {
     if (str.find("fdk")) {
@@ -124,7 +228,7 @@ Reconstructor *ReconstrctorAbstractFactory::createReconstructor(const std::strin
         else if (has(s, "shepp")) win = SmoothWindow::SHEPP_LOGAN;
         const bool piori = has(s, "piori");
         const bool null_rebin = has(s, "null");
-        if (has(s, "resident") || has(s, "f16")) {
+        if (has(s, "resident") || has(s, "f16") || has(s, "slab") || has(s, "ckpt")) {
             lct_recon_opts o{};
             o.pipeline = LCT_PIPE_RESIDENT;
             o.rebin_kind = piori ? LCT_REBIN_PI_ORI : (null_rebin ? LCT_REBIN_NULL : LCT_REBIN_FAN2PAR);
@@ -134,7 +238,14 @@ Reconstructor *ReconstrctorAbstractFactory::createReconstructor(const std::strin
             o.storage = has(s, "bf16") ? LCT_BF16 : (has(s, "f16") ? LCT_F16 : LCT_F32);
             o.filter_algo = o.storage == LCT_F32 ? LCT_FILTER_DIRECT_F32 : LCT_FILTER_TOEPLITZ_BF16X3;
             o.interp = LCT_INTERP_LDS;
-            return new FDKResidentReconstructor(o);
+            auto *r = new FDKResidentReconstructor(o);
+            const bool ckpt = has(s, "ckpt");
+            int per = 0;
+            const size_t k = s.find("slab");
+            if (k != std::string::npos) per = std::max(1, std::atoi(s.c_str() + k + 4));
+            if (ckpt && per == 0) per = 1;
+            r->set_streaming(per, ckpt);
+            return r;
         }
         Rebiner *rbn = nullptr;
         PreWeighting *pw = nullptr;

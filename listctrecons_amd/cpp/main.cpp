@@ -4,6 +4,7 @@
 // The reconstructor string defaults to the prm's optional ReconType key, then
 // to the reference's "Helicalfdkramprebingpu".
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <string>
@@ -54,16 +55,33 @@ int main(int argc, char **argv)
         return 0;
     }
     const std::string prmFile = argv[1];
+    // optional: [reconstructor string] [--stop-after-slabs N]
+    std::string arg_type;
+    int stop_after = 0;
+    for (int i = 2; i < argc; i++) {
+        if (std::strcmp(argv[i], "--stop-after-slabs") == 0 && i + 1 < argc)
+            stop_after = std::atoi(argv[++i]);
+        else
+            arg_type = argv[i];
+    }
     std::cout << "Reading prm file..." << std::endl;
     FileParams *fp = new FileParams(prmFile);
     std::cout << "Converting parameters..." << std::endl;
     ReconData *mr = new ReconData(*fp);
     std::string recon_type = "Helicalfdkramprebingpu";
     if (!fp->recon_type().empty()) recon_type = fp->recon_type();
-    if (argc > 2) recon_type = argv[2];
+    if (!arg_type.empty()) recon_type = arg_type;
     std::cout << "Creating reconstructor..." << std::endl;
     ReconstrctorAbstractFactory *rFac = new ReconstrctorAbstractFactory();
     Reconstructor *recon = rFac->createReconstructor(recon_type);
+    if (stop_after > 0) {
+        auto *res = dynamic_cast<FDKResidentReconstructor *>(recon);
+        if (!res) {
+            std::cout << "--stop-after-slabs needs a streamed (slab / ckpt) reconstructor" << std::endl;
+            return 1;
+        }
+        res->set_streaming(res->slabs_per_step(), res->checkpointing(), stop_after);
+    }
     recon->reconstruct(mr);
     ImageWriter *imWriter = new RawImageWriter();
     imWriter->writeImage(mr);

@@ -17,7 +17,7 @@ namespace lct {
 
 int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opts &o,
                          const float *h_angles, const float *h_couch, const float *d_fan,
-                         long file_views, float *d_vol);
+                         long file_views, int b0, int b1, float *d_vol);
 
 // Stage timer: records events on the context stream when profiling is on.
 struct StageClock {
@@ -52,9 +52,10 @@ long long count_updates(const lct_plan &plan)
     return per_column * plan.ig.nx * plan.ig.ny * plan.n_blocks;
 }
 
+// Blocks [b0, b1); block b's slab lands at d_vol + (b - b0) slabs.
 static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opts &o,
                               const float *h_angles, const float *h_couch, const float *d_fan,
-                              long file_views, float *d_vol)
+                              long file_views, int b0, int b1, float *d_vol)
 {
     const lct_ct_geom &g = plan.ctg;
     const lct_img_geom &ig = plan.ig;
@@ -73,7 +74,7 @@ static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_reco
     std::vector<float> betas(nb);
     StageClock clk(ctx);
     lct_timings &t = ctx->timings;
-    for (int b = 0; b < plan.n_blocks; b++) {
+    for (int b = b0; b < b1; b++) {
         int bps = 0;
         LCT_CHECK(lct_block_info(&plan, h_couch, b, &bps, nullptr, nullptr, nullptr, nullptr));
         const long first = long(bps) + plan.data_offset;
@@ -98,7 +99,7 @@ static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_reco
         }
         LCT_CHECK(clk.start());
         LCT_CHECK(backproject_block_public(ctx, o.bp_kind, g, ig, par, betas.data(),
-                                           d_vol + size_t(b) * slab));
+                                           d_vol + size_t(b - b0) * slab));
         LCT_CHECK(clk.stop(&t.backproject_ms));
         t.n_backproject_launches++;
     }
@@ -109,36 +110,62 @@ static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_reco
 
 using namespace lct;
 
-extern "C" int lct_reconstruct(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
-                               const float *h_angles, const float *h_couch, const float *d_fan,
-                               long file_views, float *d_vol)
+// Blocks [b0, b1) of the plan; d_fan[0] is projection-file view fan_view0.
+static int reconstruct_range(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
+                             const float *h_angles, const float *h_couch, const float *d_fan,
+                             long fan_view0, long fan_views, int b0, int b1, float *d_vol,
+                             const char *who)
 {
-    if (!ctx || !plan || !opts || !h_angles || !h_couch || !d_fan || !d_vol || file_views <= 0)
-        return set_error(LCT_ERR_INVALID, "lct_reconstruct: NULL argument");
+    if (!ctx || !plan || !opts || !h_angles || !h_couch || !d_fan || !d_vol || fan_views <= 0 ||
+        fan_view0 < 0)
+        return set_error(LCT_ERR_INVALID, std::string(who) + ": NULL argument");
     const lct_recon_opts &o = *opts;
     if (o.rebin_kind < 0 || o.rebin_kind > 2 || o.weight_kind < 0 || o.weight_kind > 1 ||
         o.window < 0 || o.window > 2 || o.bp_kind < 0 || o.bp_kind > 1 || o.storage < 0 ||
         o.storage > 2 || o.filter_algo < 0 || o.filter_algo > 1 || o.interp < 0 || o.interp > 1)
-        return set_error(LCT_ERR_INVALID, "lct_reconstruct: option out of range");
+        return set_error(LCT_ERR_INVALID, std::string(who) + ": option out of range");
     if (plan->n_blocks <= 0 || plan->ig.nz != LCT_BLOCK_SLICES || plan->ctg.n_angle <= 0)
-        return set_error(LCT_ERR_INVALID, "lct_reconstruct: plan has no blocks");
+        return set_error(LCT_ERR_INVALID, std::string(who) + ": plan has no blocks");
+    if (b0 < 0 || b1 <= b0 || b1 > plan->n_blocks)
+        return set_error(LCT_ERR_INVALID, std::string(who) + ": block range outside the plan");
+    // index the device buffer from its first view
+    lct_plan p = *plan;
+    p.data_offset = int(long(plan->data_offset) - fan_view0);
     LCT_CHECK_HIP(hipSetDevice(ctx->device));
     ctx->timings = lct_timings{};
     hipEvent_t t0 = ctx->ev[4], t1 = ctx->ev[5];
     if (ctx->profiling) LCT_CHECK_HIP(hipEventRecord(t0, ctx->stream));
     int rc;
     if (o.pipeline == LCT_PIPE_RESIDENT)
-        rc = reconstruct_resident(ctx, *plan, o, h_angles, h_couch, d_fan, file_views, d_vol);
+        rc = reconstruct_resident(ctx, p, o, h_angles, h_couch, d_fan, fan_views, b0, b1, d_vol);
     else if (o.pipeline == LCT_PIPE_BLOCKS)
-        rc = reconstruct_blocks(ctx, *plan, o, h_angles, h_couch, d_fan, file_views, d_vol);
+        rc = reconstruct_blocks(ctx, p, o, h_angles, h_couch, d_fan, fan_views, b0, b1, d_vol);
     else
-        rc = set_error(LCT_ERR_INVALID, "lct_reconstruct: unknown pipeline");
+        rc = set_error(LCT_ERR_INVALID, std::string(who) + ": unknown pipeline");
     if (rc != LCT_OK) return rc;
-    ctx->timings.backproject_updates = count_updates(*plan);
+    ctx->timings.backproject_updates = count_updates(p) / p.n_blocks * (b1 - b0);
     if (ctx->profiling) {
         LCT_CHECK_HIP(hipEventRecord(t1, ctx->stream));
         LCT_CHECK_HIP(hipEventSynchronize(t1));
         LCT_CHECK_HIP(hipEventElapsedTime(&ctx->timings.total_ms, t0, t1));
     }
     return LCT_OK;
+}
+
+extern "C" int lct_reconstruct(lct_ctx *ctx, const lct_plan *plan, const lct_recon_opts *opts,
+                               const float *h_angles, const float *h_couch, const float *d_fan,
+                               long file_views, float *d_vol)
+{
+    if (!plan) return set_error(LCT_ERR_INVALID, "lct_reconstruct: NULL argument");
+    return reconstruct_range(ctx, plan, opts, h_angles, h_couch, d_fan, 0, file_views, 0,
+                             plan->n_blocks, d_vol, "lct_reconstruct");
+}
+
+extern "C" int lct_reconstruct_slabs(lct_ctx *ctx, const lct_plan *plan,
+                                     const lct_recon_opts *opts, const float *h_angles,
+                                     const float *h_couch, const float *d_fan, long fan_view0,
+                                     long fan_views, int block_begin, int block_end, float *d_vol)
+{
+    return reconstruct_range(ctx, plan, opts, h_angles, h_couch, d_fan, fan_view0, fan_views,
+                             block_begin, block_end, d_vol, "lct_reconstruct_slabs");
 }

@@ -207,3 +207,25 @@ extern "C" int lct_block_info(const lct_plan *plan, const float *couch, int bloc
     if (ref_view) *ref_view = s + half_turn + g.add_projections;
     return LCT_OK;
 }
+
+extern "C" int lct_slab_views(const lct_plan *plan, const float *couch, int block_begin,
+                              int block_end, long *view_begin, long *view_end)
+{
+    using lct::set_error;
+    if (!plan || !couch || !view_begin || !view_end || block_begin < 0 ||
+        block_end <= block_begin || block_end > plan->n_blocks)
+        return set_error(LCT_ERR_INVALID, "lct_slab_views: bad argument");
+    long lo = 0, hi = 0;
+    for (int b = block_begin; b < block_end; b++) {
+        int s = 0;
+        const int rc = lct_block_info(plan, couch, b, &s, nullptr, nullptr, nullptr, nullptr);
+        if (rc != LCT_OK) return rc;
+        const long first = long(s) + plan->data_offset;
+        const long last = first + plan->ctg.n_angle_add;
+        lo = b == block_begin ? first : std::min(lo, first);
+        hi = b == block_begin ? last : std::max(hi, last);
+    }
+    *view_begin = lo;
+    *view_end = hi;
+    return LCT_OK;
+}

@@ -562,21 +562,23 @@ static int tile_half_width(const lct_ct_geom &g, const lct_img_geom &ig)
     return int(std::ceil(half_extent * slope / g.ds)) + 1;
 }
 
+// Parallel views [g0, g0 + nviews) that blocks [b0, b1) backproject, and
+// each block's blockProjStart (bps[b - b0]).
 struct Resident {
     long g0 = 0, nviews = 0;
     std::vector<int> bps;
 };
 
-static int plan_resident(const lct_plan &plan, const float *h_couch, long file_views,
-                         Resident &r)
+static int plan_resident(const lct_plan &plan, const float *h_couch, long file_views, int b0,
+                         int b1, Resident &r)
 {
     const lct_ct_geom &g = plan.ctg;
-    r.bps.resize(plan.n_blocks);
+    r.bps.resize(size_t(b1 - b0));
     long lo = 1L << 40, hi = -(1L << 40);
-    for (int b = 0; b < plan.n_blocks; b++) {
+    for (int b = b0; b < b1; b++) {
         int s = 0;
         LCT_CHECK(lct_block_info(&plan, h_couch, b, &s, nullptr, nullptr, nullptr, nullptr));
-        r.bps[b] = s;
+        r.bps[size_t(b - b0)] = s;
         if (s < 0 || long(s) + plan.data_offset < 0 ||
             long(s) + plan.data_offset + g.n_angle_add > file_views ||
             s + g.add_projections + g.n_angle > plan.n_readings)
@@ -592,12 +594,13 @@ static int plan_resident(const lct_plan &plan, const float *h_couch, long file_v
 
 int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opts &o,
                          const float *h_angles, const float *h_couch, const float *d_fan,
-                         long file_views, float *d_vol)
+                         long file_views, int b0, int b1, float *d_vol)
 {
     const lct_ct_geom &g = plan.ctg;
     const lct_img_geom &ig = plan.ig;
+    const int nb = b1 - b0;
     Resident R;
-    LCT_CHECK(plan_resident(plan, h_couch, file_views, R));
+    LCT_CHECK(plan_resident(plan, h_couch, file_views, b0, b1, R));
     const size_t view = size_t(g.nt) * g.ns;
     const int h = g.n_proj_turn / 2;
 
@@ -668,15 +671,15 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         LCT_CHECK(toc(&t.filter_ms));
         const size_t slab = size_t(ig.nx) * ig.ny * ig.nz;
         std::vector<float> betas(g.n_angle + h + 1);
-        for (int b = 0; b < plan.n_blocks; b++) {
-            const long off = long(R.bps[b]) + g.add_projections - R.g0;
+        for (int k = 0; k < nb; k++) {
+            const long off = long(R.bps[k]) + g.add_projections - R.g0;
             for (size_t i = 0; i < betas.size(); i++) {
-                const long src = long(R.bps[b]) + g.add_projections + long(i);
+                const long src = long(R.bps[k]) + g.add_projections + long(i);
                 betas[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
             }
             LCT_CHECK(tic());
             LCT_CHECK(backproject_block_public(ctx, o.bp_kind, g, ig, P + size_t(off) * view,
-                                               betas.data(), d_vol + size_t(b) * slab));
+                                               betas.data(), d_vol + size_t(k) * slab));
             LCT_CHECK(toc(&t.backproject_ms));
             t.n_backproject_launches++;
         }
@@ -708,11 +711,11 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     LCT_CHECK(toc(&t.filter_ms));
 
     // 3. backproject every block from Q in one launch.
-    std::vector<int> tab(34 * size_t(plan.n_blocks));
-    for (int b = 0; b < plan.n_blocks; b++) {
-        tab[34 * b] = int(long(R.bps[b]) + g.add_projections - R.g0);
-        tab[34 * b + 1] = b * LCT_BLOCK_SLICES;
-        for (int z = 0; z < 32; z++) tab[34 * b + 2 + z] = amid[z];
+    std::vector<int> tab(34 * size_t(nb));
+    for (int k = 0; k < nb; k++) {
+        tab[34 * k] = int(long(R.bps[k]) + g.add_projections - R.g0);
+        tab[34 * k + 1] = k * LCT_BLOCK_SLICES;
+        for (int z = 0; z < 32; z++) tab[34 * k + 2 + z] = amid[z];
     }
     // slices whose window holds pair j (a_middle is the same in every block)
     std::vector<unsigned> zmask(size_t(amid[31] - amid[0] + h));
@@ -771,17 +774,17 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     LCT_CHECK(tic());
     if (o.interp == LCT_INTERP_GATHER) {
         if (o.storage == LCT_F32)
-            LCT_CHECK(launch_bp_fold_gather<float>(ctx, a, plan.n_blocks));
+            LCT_CHECK(launch_bp_fold_gather<float>(ctx, a, nb));
         else if (o.storage == LCT_F16)
-            LCT_CHECK(launch_bp_fold_gather<__half>(ctx, a, plan.n_blocks));
+            LCT_CHECK(launch_bp_fold_gather<__half>(ctx, a, nb));
         else
-            LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, a, plan.n_blocks));
+            LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, a, nb));
     } else if (o.storage == LCT_F32)
-        LCT_CHECK(launch_bp_fold<float>(ctx, a, plan.n_blocks));
+        LCT_CHECK(launch_bp_fold<float>(ctx, a, nb));
     else if (o.storage == LCT_F16)
-        LCT_CHECK(launch_bp_fold<__half>(ctx, a, plan.n_blocks));
+        LCT_CHECK(launch_bp_fold<__half>(ctx, a, nb));
     else
-        LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, a, plan.n_blocks));
+        LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, a, nb));
     LCT_CHECK(toc(&t.backproject_ms));
     t.n_backproject_launches++;
     return LCT_OK;

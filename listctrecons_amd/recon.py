@@ -53,6 +53,16 @@ def block_info(plan: _lib.Plan, couch: np.ndarray, block: int):
     return s.value, e.value, bs.value, be.value, r.value
 
 
+def slab_views(plan: _lib.Plan, couch: np.ndarray, block_begin: int, block_end: int):
+    """Projection-file views [begin, end) that blocks [block_begin, block_end)
+    read (the union of their BlockInfo windows, DataOffset applied)."""
+    c = np.ascontiguousarray(couch, np.float32)
+    vb, ve = C.c_long(), C.c_long()
+    check(lib().lct_slab_views(C.byref(plan), _fptr(c), block_begin, block_end, C.byref(vb),
+                               C.byref(ve)))
+    return vb.value, ve.value
+
+
 def ramp_taps(window: int, ns: int, ds: float) -> np.ndarray:
     """RampFilterGPU::calculate_filter (Src/Filter.cpp:8-20) + apodisation."""
     h = np.empty(2 * ns, np.float32)
@@ -201,6 +211,26 @@ class Context:
         oc = opts.to_c()
         check(lib().lct_reconstruct(self.h, C.byref(plan), C.byref(oc), _fptr(a), _fptr(c),
                                     _dev(fan), views, _dev(vol)))
+        return vol
+
+    def reconstruct_slabs(self, plan: _lib.Plan, opts: Options, angles: np.ndarray,
+                          couch: np.ndarray, fan, fan_view0: int, block_begin: int,
+                          block_end: int, vol=None):
+        """Blocks [block_begin, block_end) only; fan holds projection-file
+        views starting at fan_view0.  Returns (block_end - block_begin) * 32
+        slices."""
+        import torch
+        a = np.ascontiguousarray(angles, np.float32)
+        c = np.ascontiguousarray(couch, np.float32)
+        g = plan.ctg
+        views = fan.numel() // (g.nt * g.ns)
+        if vol is None:
+            vol = torch.empty(((block_end - block_begin) * 32, plan.ig.ny, plan.ig.nx),
+                              dtype=torch.float32, device=fan.device)
+        oc = opts.to_c()
+        check(lib().lct_reconstruct_slabs(self.h, C.byref(plan), C.byref(oc), _fptr(a), _fptr(c),
+                                          _dev(fan), fan_view0, views, block_begin, block_end,
+                                          _dev(vol)))
         return vol
 
     # -------------------------------------------------------- synthetic data

@@ -51,6 +51,25 @@ def test_host_planner_without_gpu():
     assert (s1 - s0) == pytest.approx(32 * w.dz / (w.pitch_value / w.n_proj_turn), abs=1.0)
 
 
+@pytest.mark.parametrize("data_offset", [0, 7])
+def test_slab_views_cover_the_block_windows(data_offset):
+    """lct_slab_views == union of the oracle's BlockInfo windows + DataOffset."""
+    from oracle import oracle as O
+    from helpers import oracle_plan
+    w = scan.workload("helical_64x736_512x256", data_offset=data_offset)
+    opl, a, c = oracle_plan(w)
+    pl = recon.plan_helical(w.prm(), a, c)
+    nb = pl.n_blocks
+    for b0, b1 in ((0, 1), (3, 4), (2, 6), (0, nb)):
+        starts = [O.block_info(opl, c, b)[0] for b in range(b0, b1)]
+        want = (min(starts) + data_offset, max(starts) + data_offset + pl.ctg.n_angle_add)
+        assert recon.slab_views(pl, c, b0, b1) == want
+    for b0, b1 in ((-1, 2), (3, 3), (0, nb + 1)):
+        with pytest.raises(_lib.LctError) as e:
+            recon.slab_views(pl, c, b0, b1)
+        assert e.value.code == _lib.LCT_ERR_INVALID
+
+
 def test_planner_range_error_code():
     w = scan.workload("tiny", readings=300)
     a, c = w.angles_couch()

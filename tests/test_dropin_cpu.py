@@ -90,3 +90,16 @@ def test_cli_errors(tmp_path):
     prm = scan.write_scan_files(w, str(tmp_path / "short"))
     r = run(CLI, "--plan", prm)
     assert r.returncode == 1 and "outside of allowed data range" in r.stdout
+
+
+def test_cli_refuses_a_foreign_checkpoint(tmp_path):
+    """A streamed run with checkpointing checks "<output>.ckpt" before any
+    device work and exits when it records another reconstruction."""
+    w = scan.workload("tiny", readings=1400, pitch=1.7)
+    prm = scan.write_scan_files(w, str(tmp_path))
+    ck = tmp_path / "image.raw.ckpt"
+    for text in ("LISTCTRecons checkpoint 1\nplan 9 9\ndone 1\n", "garbage\n"):
+        ck.write_text(text)
+        r = run(CLI, prm, "Helicalfdkramprebingpu-resident-ckpt")
+        assert r.returncode == 1
+        assert "belongs to another reconstruction" in r.stdout

@@ -75,3 +75,52 @@ def test_factory_rejects_strings_starting_with_fdk(case):
                        text=True, timeout=120)
     assert r.returncode == 1
     assert "Error happens when building reconstructor." in r.stdout
+
+
+@pytest.fixture(scope="module")
+def three_blocks(tmp_path_factory):
+    w = scan.workload("tiny", readings=1400, pitch=1.7, data_offset=4)
+    fan = cpu_fan(w)
+    dirs = {}
+    for k in ("whole", "slab2", "resumed"):
+        d = str(tmp_path_factory.mktemp(k))
+        dirs[k] = (d, scan.write_scan_files(w, d, fan))
+    pl, a, c = oracle_plan(w)
+    assert pl.n_blocks == 3
+    ref = O.reconstruct(pl, fan, a, c, window=w.window)
+    return w, dirs, ref
+
+
+def test_cli_slab_streaming_and_resume(three_blocks):
+    """Streaming by slabs, and a checkpointed run stopped after one slab and
+    resumed, write the same image.raw bit for bit as the one-shot resident
+    run; it matches the oracle."""
+    w, dirs, ref = three_blocks
+
+    def cli(key, *extra):
+        d, prm = dirs[key]
+        print(f"CLI {key} {extra}", flush=True)
+        return subprocess.run([CLI, prm, *extra], stdout=subprocess.PIPE,
+                              stderr=subprocess.STDOUT, text=True, timeout=300)
+
+    def image(key):
+        return np.fromfile(os.path.join(dirs[key][0], "image.raw"), np.float32)
+
+    r = cli("whole", "Helicalfdkramprebingpu-resident")
+    assert r.returncode == 0, r.stdout[-2000:]
+    whole = image("whole")
+    assert rel_rmse(whole.reshape(ref.shape), ref) <= 1e-4
+
+    r = cli("slab2", "Helicalfdkramprebingpu-resident-slab2")
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert np.array_equal(image("slab2"), whole)
+
+    ck = os.path.join(dirs["resumed"][0], "image.raw.ckpt")
+    r = cli("resumed", "Helicalfdkramprebingpu-resident-ckpt", "--stop-after-slabs", "1")
+    assert r.returncode == 3, r.stdout[-2000:]
+    assert open(ck).read().splitlines()[-1] == "done 1"
+    r = cli("resumed", "Helicalfdkramprebingpu-resident-ckpt")
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "Resuming after block 0" in r.stdout
+    assert not os.path.exists(ck)
+    assert np.array_equal(image("resumed"), whole)

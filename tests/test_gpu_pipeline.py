@@ -156,6 +156,28 @@ def test_clinical_sampled_parity(ctx, clinical, clinical_ref, preset):
         assert rmse_hu <= 1.0
 
 
+@pytest.mark.parametrize("preset", ["plain", "fast_f32", "mixed_f16"])
+def test_slabs_equal_the_whole_volume(ctx, clinical, preset):
+    """lct_reconstruct_slabs from a buffer holding only the slab's views gives
+    the same slices, bit for bit, as the whole-volume reconstruction."""
+    w, fan, a, c = clinical
+    print(f"slabs {preset}", flush=True)
+    pl = recon.plan_helical(w.prm(), a, c)
+    opts = recon.preset(preset, window=w.window)
+    whole = ctx.reconstruct(pl, opts, a, c, fan).clone()
+    for b0, b1 in ((0, 1), (2, 5), (7, 8)):
+        v0, v1 = recon.slab_views(pl, c, b0, b1)
+        part = fan[v0:v1].contiguous()
+        got = ctx.reconstruct_slabs(pl, opts, a, c, part, v0, b0, b1)
+        ctx.synchronize()
+        assert torch.equal(got, whole[32 * b0:32 * b1]), (b0, b1)
+    # a buffer that lacks the slab's first view is refused
+    v0, v1 = recon.slab_views(pl, c, 2, 5)
+    with pytest.raises(_lib.LctError) as e:
+        ctx.reconstruct_slabs(pl, opts, a, c, fan[v0 + 1:v1].contiguous(), v0 + 1, 2, 5)
+    assert e.value.code == _lib.LCT_ERR_RANGE
+
+
 def test_clinical_linearity_full_volume(ctx, clinical):
     """reconstruct(2 p) == 2 reconstruct(p) over the whole 512^2 x 256 volume."""
     w, fan, a, c = clinical
