@@ -155,6 +155,21 @@ def main():
     head_vol = vol.cpu().numpy()
     log(f"  {head['s_per_volume']*1e3:.2f} ms/volume  stages {head['stage_ms']}")
 
+    # host -> device upload of the projection file from pinned memory: what a
+    # caller that holds the scan in host memory pays on top of `value`
+    host_fan = torch.empty(fan.shape, dtype=fan.dtype, pin_memory=True)
+    host_fan.copy_(fan)
+    dev_fan = torch.empty_like(fan)
+    h2d = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev_fan.copy_(host_fan, non_blocking=True)
+        torch.cuda.synchronize()
+        h2d.append(time.perf_counter() - t0)
+    h2d_s = min(h2d)
+    del host_fan, dev_fan
+
     plain = None
     plain_vol = None
     if not args.no_plain:
@@ -226,6 +241,10 @@ def main():
                           f"per launch; the taps are read from LDS, so frac > 1 is possible",
         },
     }
+    out["pcie"] = {"h2d_ms": round(h2d_s * 1e3, 3),
+                   "h2d_GBps": round(fan.numel() * 4 / h2d_s / 1e9, 1),
+                   "s_per_volume_incl_h2d": round(head["s_per_volume"] + h2d_s, 6),
+                   "note": "projection file uploaded from pinned host memory; not in value"}
     if variants:
         out["variants"] = variants
     if plain:

@@ -144,6 +144,16 @@ void lco_project_ellipsoids(const lco_ct_geom *g, const float *angles, const flo
 void lco_voxelize(const lco_img_geom *ig, double z0, int nz, const double *ell, int n_ell,
                   int supersample, double *vol);
 
+/* Reprojection of a voxel volume vol[nz][ny][nx] (voxel centres as above,
+ * trilinear interpolation, zero outside) along the rays of
+ * lco_project_ellipsoids: midpoint rule over n = ceil(L / step) equal steps of
+ * the ray's chord L through the box of voxel centres.  Not in the reference:
+ * the definition the build's lct_forward_project implements (reprojection
+ * residual of the learned correction).  out [n_views][nt][ns]. */
+void lco_forward_project(const lco_ct_geom *g, const float *angles, const float *couch,
+                         long view0, long n_views, const double *vol, const lco_img_geom *ig,
+                         double z0, int nz, double step, double *out);
+
 int lco_num_threads(void);
 void lco_set_num_threads(int n);
 

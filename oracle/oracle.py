@@ -111,6 +111,8 @@ def lib():
                                              f64p, C.c_int, f64p]
         L.lco_voxelize.argtypes = [P(ImgGeom), C.c_double, C.c_int, f64p, C.c_int, C.c_int,
                                    f64p]
+        L.lco_forward_project.argtypes = [P(CTGeom), f32p, f32p, C.c_long, C.c_long, f64p,
+                                          P(ImgGeom), C.c_double, C.c_int, C.c_double, f64p]
         L.lco_num_threads.restype = C.c_int
         L.lco_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -249,6 +251,21 @@ def project_ellipsoids(g: CTGeom, angles: np.ndarray, couch: np.ndarray, ell: np
     out = np.empty((n_views, g.nt, g.ns), dtype=np.float64)
     lib().lco_project_ellipsoids(C.byref(g), _p(a, C.c_float), _p(c, C.c_float), view0,
                                  n_views, _p(e, C.c_double), e.shape[0], _p(out, C.c_double))
+    return out
+
+
+def forward_project(g: CTGeom, angles: np.ndarray, couch: np.ndarray, vol: np.ndarray,
+                    ig: ImgGeom, z0: float, step: float, view0: int = 0,
+                    n_views: int | None = None) -> np.ndarray:
+    """Reprojection of vol [nz][ny][nx] (trilinear, midpoint rule)."""
+    a, c = _f32(angles), _f32(couch)
+    v = _f64(vol)
+    if n_views is None:
+        n_views = a.size - view0
+    out = np.empty((n_views, g.nt, g.ns), dtype=np.float64)
+    lib().lco_forward_project(C.byref(g), _p(a, C.c_float), _p(c, C.c_float), view0, n_views,
+                              _p(v, C.c_double), C.byref(ig), float(z0), v.shape[0],
+                              float(step), _p(out, C.c_double))
     return out
 
 
