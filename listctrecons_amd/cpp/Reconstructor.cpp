@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include <sys/stat.h>
 #include <sys/types.h>
 #include <unistd.h>
 
@@ -142,22 +143,30 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
         }
     }
 
-    // device buffers sized for the largest slab
-    long max_views = 0;
+    // device buffers sized for the largest slab; the projection file must hold
+    // every view the remaining slabs read (checked before any device work)
+    const std::string in_path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
+    long max_views = 0, last_view = 0;
     for (int b0 = start; b0 < plan.n_blocks; b0 += per) {
         long v0 = 0, v1 = 0;
         if (lct_slab_views(&plan, couch, b0, std::min(b0 + per, plan.n_blocks), &v0, &v1) != LCT_OK)
             fail(lct_last_error());
         if (v0 < 0) fail("projection window starts before the data file");
         max_views = std::max(max_views, v1 - v0);
+        last_view = std::max(last_view, v1);
     }
+    struct stat sb;
+    if (stat(in_path.c_str(), &sb) != 0) fail("raw data file " + in_path + " not found");
+    if (off_t(sb.st_size) < off_t(last_view) * off_t(view) * off_t(sizeof(float)))
+        fail("projection file " + in_path + " holds " +
+             std::to_string(sb.st_size / (off_t(view) * off_t(sizeof(float)))) +
+             " views; the planned blocks read up to view " + std::to_string(last_view));
     GPUMemoryCtrl<float> fan, vol;
     if (start < plan.n_blocks &&
         (!fan.allocate_memory(view * size_t(max_views)) ||
          !vol.allocate_memory(size_t(per) * BLOCK_SLICES * slice)))
         fail(lct_last_error());
 
-    const std::string in_path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
     std::vector<float> host;
     int run = 0;
     for (int b0 = start; b0 < plan.n_blocks; b0 += per) {

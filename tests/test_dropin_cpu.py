@@ -92,6 +92,26 @@ def test_cli_errors(tmp_path):
     assert r.returncode == 1 and "outside of allowed data range" in r.stdout
 
 
+@pytest.mark.parametrize("recon_type,keep", [
+    ("Helicalfdkramprebingpu", 1000),                   # block 0's window is cut
+    ("Helicalfdkramprebingpu-resident", None),          # half the file: later blocks cut
+    ("Helicalfdkramprebingpu-resident-slab1-ckpt", None),
+])
+def test_cli_truncated_projection_file(tmp_path, recon_type, keep):
+    """A projection file shorter than the planned windows exits 1 with a
+    message before any device work (the reference read it unchecked,
+    Src/DataReader.cpp:12-18)."""
+    w = scan.workload("tiny", readings=1400, pitch=1.7)
+    prm = scan.write_scan_files(w, str(tmp_path))
+    raw = tmp_path / "proj.raw"
+    size = raw.stat().st_size
+    os.truncate(raw, keep if keep is not None else size // 2)
+    r = run(CLI, prm, recon_type)
+    assert r.returncode == 1, r.stdout[-1500:]
+    assert "does not hold views" in r.stdout or "holds" in r.stdout, r.stdout[-1500:]
+    assert "hip" not in r.stdout.lower()
+
+
 def test_cli_refuses_a_foreign_checkpoint(tmp_path):
     """A streamed run with checkpointing checks "<output>.ckpt" before any
     device work and exits when it records another reconstruction."""
