@@ -5,9 +5,11 @@
 A step is one complete reconstruction (rebin -> filter -> backproject of all
 planned blocks) of the 512 x 512 x 256 clinical workload from 8192 views of a
 64 x 736 detector that are already resident in HBM.  The headline path is
-the mixed-precision one (f16 filtered projections, FP32 accumulation); the
-plain FP32 path (the reference's per-block loop on the GPU) is timed beside
-it, and the FP64 oracle is timed on the host on a bounded sample.
+the mixed-precision one (f16 filtered projections, FP32 accumulation); bf16
+storage and the plain FP32 path (the reference's per-block loop on the GPU)
+are timed beside it, and the FP64 oracle is timed on the host on a bounded
+sample.  Optional: slab streaming from pinned host memory (--stream-slabs),
+the learned correction's inference (--no-learned to skip).
 
 Prints one JSON line (rank 0).
 """
@@ -63,15 +65,28 @@ def time_path(ctx, pl, opts, a, c, fan, vol, steps, warmup):
     }
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, fan_np, a, c, sample_slices=(0, 11, 22)):
     """FP64 oracle on the host: block 0 (rebin + filter of its whole window, and
-    `sample_slices` backprojected), scaled to a whole volume."""
+    `sample_slices` backprojected), scaled to a whole volume, on all host
+    threads; and a single-thread figure from smaller sub-samples."""
     from oracle import oracle as O
     p = O.Prm()
     pp = w.prm()
     for f, _ in O.Prm._fields_:
         setattr(p, f, getattr(pp, f))
     pl = O.plan_helical(p, a, c)
+    threads = O.num_threads()
     t0 = time.perf_counter()
     O.reconstruct(pl, fan_np, a, c, window=w.window, blocks=(0, 1), iz_list=[])
     t_rf = time.perf_counter() - t0
@@ -81,17 +96,105 @@ def cpu_baseline(w, fan_np, a, c, sample_slices=(0, 11, 22)):
     t_all = time.perf_counter() - t0
     t_slice = max(t_all - t_rf, 0.0) / len(sample_slices)
     per_volume = pl.n_blocks * (t_rf + 32 * t_slice)
+
+    # single thread: rebin of block 0's window, filter of 256 of its rows,
+    # 64 rows of one slice backprojected; each scaled to the volume
+    g, ig = pl.ctg, pl.ig
+    s, _, _, _ = O.block_info(pl, c, 0)
+    first = s + pl.data_offset
+    win = fan_np[first:first + g.n_angle_add]
+    taps = O.ramp_taps(g.ns, g.ds, w.window)
+    O.set_num_threads(1)
+    try:
+        t0 = time.perf_counter()
+        par = O.rebin(g, win, kind=1)
+        t_rebin1 = time.perf_counter() - t0
+        rows = par.reshape(-1, g.ns)
+        t0 = time.perf_counter()
+        O.filter_rows(taps, rows[:256])
+        t_filter1 = (time.perf_counter() - t0) * rows.shape[0] / 256
+        # the backprojection's cost does not depend on the values it sums
+        betas = a[s + g.add_projections:s + g.add_projections + g.n_angle + g.n_proj_turn // 2 + 1]
+        t0 = time.perf_counter()
+        O.backproject(g, ig, par, betas, iz_list=[11], iy_range=(224, 288))
+        t_slice1 = (time.perf_counter() - t0) * ig.ny / 64
+    finally:
+        O.set_num_threads(threads)
+    per_volume_1 = pl.n_blocks * (t_rebin1 + t_filter1 + 32 * t_slice1)
     return {
         "value": per_volume,
         "unit": "s/volume",
-        "cores": O.num_threads(),
+        "cores": threads,
         "kind": "port",
-        "sample": (f"FP64 oracle, block 0 of {pl.n_blocks}: rebin+filter of its "
-                   f"{pl.ctg.n_angle_add}-view window ({t_rf:.2f} s) and {len(sample_slices)} of 32 "
-                   f"slices backprojected ({t_slice:.2f} s/slice), scaled to "
+        "cpu": cpu_model(),
+        "nproc": os.cpu_count(),
+        "sample": (f"FP64 oracle, {threads} threads, block 0 of {pl.n_blocks}: rebin+filter of its "
+                   f"{pl.ctg.n_angle_add}-view window ({t_rf:.2f} s) and {len(sample_slices)} of "
+                   f"32 slices backprojected ({t_slice:.2f} s/slice), scaled to "
                    f"{pl.n_blocks} blocks x 32 slices"),
-        "sample_seconds": t_rf + t_all,
+        "sample_seconds": round(t_rf + t_all, 3),
+        "single_thread": {
+            "value": per_volume_1, "unit": "s/volume",
+            "sample": (f"1 thread: rebin of block 0's window ({t_rebin1:.2f} s), filter of 256 "
+                       f"rows scaled to {rows.shape[0]} ({t_filter1:.1f} s), 64 rows of slice 11 "
+                       f"scaled to {ig.ny} rows ({t_slice1:.1f} s/slice); x 32 slices x "
+                       f"{pl.n_blocks} blocks")},
     }, vol, sample_slices
+
+
+def stream_slabs(ctx, pl, opts, a, c, host_fan, vol, blocks_per_slab, reps=2):
+    """Reconstruct slab by slab from pinned host memory: each slab's views are
+    uploaded on a copy stream (double buffered) while the previous slab is
+    backprojected.  Returns seconds per volume (best of `reps`)."""
+    import torch
+    from listctrecons_amd import recon
+    g = pl.ctg
+    view = g.nt * g.ns
+    slabs = []
+    for b0 in range(0, pl.n_blocks, blocks_per_slab):
+        b1 = min(b0 + blocks_per_slab, pl.n_blocks)
+        slabs.append((b0, b1) + recon.slab_views(pl, c, b0, b1))
+    max_views = max(v1 - v0 for _, _, v0, v1 in slabs)
+    bufs = [torch.empty(max_views * view, dtype=torch.float32, device="cuda") for _ in range(2)]
+    # two non-default streams (default priority): reconstruction and copies
+    comp = torch.cuda.Stream()
+    copy = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(comp.cuda_stream)
+    flat = host_fan.view(-1)
+    best = None
+    try:
+        for _ in range(reps):
+            copied = [torch.cuda.Event(), torch.cuda.Event()]
+            freed = [torch.cuda.Event(), torch.cuda.Event()]
+            for e in freed:
+                e.record(comp)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+
+            def issue(k):
+                b0, b1, v0, v1 = slabs[k]
+                with torch.cuda.stream(copy):
+                    copy.wait_event(freed[k % 2])
+                    bufs[k % 2][:(v1 - v0) * view].copy_(flat[v0 * view:v1 * view],
+                                                         non_blocking=True)
+                    copied[k % 2].record(copy)
+
+            issue(0)
+            for k, (b0, b1, v0, v1) in enumerate(slabs):
+                if k + 1 < len(slabs):
+                    issue(k + 1)
+                comp.wait_event(copied[k % 2])
+                part = bufs[k % 2][:(v1 - v0) * view].view(v1 - v0, g.nt, g.ns)
+                ctx.reconstruct_slabs(pl, opts, a, c, part, v0, b0, b1,
+                                      vol[32 * b0:32 * b1])
+                freed[k % 2].record(comp)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+    finally:
+        ctx.set_stream(0)
+    return best, len(slabs), max_views
 
 
 def load_traffic(kernel_tag):
@@ -105,6 +208,12 @@ def load_traffic(kernel_tag):
         return None
 
 
+def hu_stats(x, ref, np, scan):
+    d = (x - ref).astype(np.float64) * scan.hu_per_unit()
+    return {"rmse": round(float(np.sqrt(np.mean(d * d))), 4),
+            "max_abs": round(float(np.max(np.abs(d))), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,9 +223,13 @@ def main():
     ap.add_argument("--path", default="mixed_f16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--variants", default="",
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip timing the learned correction's inference")
+    ap.add_argument("--variants", default="mixed_bf16",
                     help="comma-separated presets to time besides the headline path "
                          "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
+    ap.add_argument("--stream-slabs", type=int, default=0,
+                    help="also time slab streaming from pinned host memory, N blocks per slab")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one device copy of this many GiB first (PMC scale reference)")
     args = ap.parse_args()
@@ -155,20 +268,41 @@ def main():
     head_vol = vol.cpu().numpy()
     log(f"  {head['s_per_volume']*1e3:.2f} ms/volume  stages {head['stage_ms']}")
 
-    # host -> device upload of the projection file from pinned memory: what a
-    # caller that holds the scan in host memory pays on top of `value`
+    # host <-> device transfers a caller holding the scan in host memory pays on
+    # top of `value`: the projection file up (pinned), the volume down
     host_fan = torch.empty(fan.shape, dtype=fan.dtype, pin_memory=True)
     host_fan.copy_(fan)
     dev_fan = torch.empty_like(fan)
-    h2d = []
+    host_vol = torch.empty(vol.shape, dtype=vol.dtype, pin_memory=True)
+    h2d, d2h = [], []
     for _ in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         dev_fan.copy_(host_fan, non_blocking=True)
         torch.cuda.synchronize()
         h2d.append(time.perf_counter() - t0)
-    h2d_s = min(h2d)
-    del host_fan, dev_fan
+        t0 = time.perf_counter()
+        host_vol.copy_(vol, non_blocking=True)
+        torch.cuda.synchronize()
+        d2h.append(time.perf_counter() - t0)
+    h2d_s, d2h_s = min(h2d), min(d2h)
+    del dev_fan, host_vol
+
+    streamed = None
+    if args.stream_slabs > 0:
+        log(f"timing slab streaming, {args.stream_slabs} blocks per slab")
+        svol = torch.empty_like(vol)
+        st_s, n_slabs, max_views = stream_slabs(ctx, pl, head_opts, a, c, host_fan, svol,
+                                                args.stream_slabs)
+        streamed = {"blocks_per_slab": args.stream_slabs, "slabs": n_slabs,
+                    "s_per_volume": round(st_s, 6),
+                    "max_views_per_slab": max_views,
+                    "identical_to_one_shot": bool(torch.equal(svol, vol)),
+                    "note": "projections streamed from pinned host memory, each slab's views "
+                            "uploaded on a copy stream while the previous slab is reconstructed"}
+        log(f"  {st_s*1e3:.1f} ms/volume streamed in {n_slabs} slabs")
+        del svol
+    del host_fan
 
     plain = None
     plain_vol = None
@@ -179,20 +313,46 @@ def main():
                           max(1, min(args.steps, 3)), 1)
         plain_vol = vol2.cpu().numpy()
         log(f"  {plain['s_per_volume']*1e3:.2f} ms/volume  stages {plain['stage_ms']}")
+        del vol2
 
-    variants = {}
-    for name in [v for v in args.variants.split(",") if v]:
+    variants, variant_vols = {}, {}
+    for name in [v for v in args.variants.split(",") if v and v != args.path]:
         log(f"timing variant {name}")
         vv = torch.empty_like(vol)
         r = time_path(ctx, pl, recon.preset(name, window=w.window), a, c, fan, vv,
                       max(1, min(args.steps, 3)), 1)
-        d = (vv.cpu().numpy() - head_vol).astype(np.float64) * scan.hu_per_unit()
+        variant_vols[name] = vv.cpu().numpy()
         variants[name] = {"s_per_volume": round(r["s_per_volume"], 6),
                           "stage_ms": {k: round(x, 3) for k, x in r["stage_ms"].items()},
                           "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"]
                                                                     * 1e-3) / 1e9, 2),
-                          "hu_rmse_vs_headline": round(float(np.sqrt(np.mean(d * d))), 4)}
+                          "hu_vs_headline": hu_stats(variant_vols[name], head_vol, np, scan)}
+        if plain_vol is not None:
+            variants[name]["hu_vs_fp32_plain"] = hu_stats(variant_vols[name], plain_vol, np, scan)
         del vv
+
+    learned = None
+    if not args.no_learned:
+        # inference cost of the learned correction on this volume (the trained
+        # network's accuracy is measured by python -m listctrecons_amd.learned)
+        from listctrecons_amd import learned as L
+        log("timing the learned correction's inference")
+        net = recon.Cnn(ctx, seed=0)
+        L.correct(net, vol)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            L.correct(net, vol)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        net.close()
+        learned = {"cnn_s_per_volume": round(min(ts), 6),
+                   "s_per_volume_with_correction": round(head["s_per_volume"] + min(ts), 6),
+                   "network": "3D residual CNN, 6 x (3x3x3 conv, 32 channels), BF16 MFMA",
+                   "note": "inference time with initial weights (the cost does not depend on "
+                           "them); trained accuracy in profiles/learned_pitch15.json"}
+        torch.cuda.empty_cache()
 
     storage = {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16",
                "mixed_f16_gather": "f16", "mixed_bf16_gather": "bf16"}.get(args.path, "f32")
@@ -215,6 +375,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(head["s_per_volume"] * 1e3, 3),
+        "median_s_per_volume": round(head["median_s"], 6),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -240,20 +401,24 @@ def main():
                           f"+ 4 B per voxel written, per launch; {updates / launches:.4g} updates "
                           f"per launch; the taps are read from LDS, so frac > 1 is possible",
         },
+        "pcie": {"h2d_ms": round(h2d_s * 1e3, 3),
+                 "h2d_GBps": round(fan.numel() * 4 / h2d_s / 1e9, 1),
+                 "d2h_ms": round(d2h_s * 1e3, 3),
+                 "s_per_volume_incl_transfers": round(head["s_per_volume"] + h2d_s + d2h_s, 6),
+                 "note": "projection file up from pinned host memory, volume down to pinned "
+                         "host memory; not in value"},
     }
-    out["pcie"] = {"h2d_ms": round(h2d_s * 1e3, 3),
-                   "h2d_GBps": round(fan.numel() * 4 / h2d_s / 1e9, 1),
-                   "s_per_volume_incl_h2d": round(head["s_per_volume"] + h2d_s, 6),
-                   "note": "projection file uploaded from pinned host memory; not in value"}
+    if streamed:
+        out["slab_streaming"] = streamed
     if variants:
         out["variants"] = variants
+    if learned:
+        out["learned"] = learned
     if plain:
         out["fp32_plain_s_per_volume"] = round(plain["s_per_volume"], 6)
         out["fp32_plain_stage_ms"] = {k: round(v, 3) for k, v in plain["stage_ms"].items()}
         out["speedup_vs_fp32_plain"] = round(plain["s_per_volume"] / head["s_per_volume"], 3)
-        d = (head_vol - plain_vol).astype(np.float64) * scan.hu_per_unit()
-        out["hu_vs_fp32_plain"] = {"rmse": round(float(np.sqrt(np.mean(d * d))), 4),
-                                   "max_abs": round(float(np.max(np.abs(d))), 3)}
+        out["hu_vs_fp32_plain"] = hu_stats(head_vol, plain_vol, np, scan)
 
     if not args.no_cpu_baseline:
         log("timing the FP64 oracle on the host (bounded sample)")
@@ -262,15 +427,15 @@ def main():
         out["cpu_baseline"] = cb
         idx = list(sl)
         ref = ovol[idx]
-        for name, v in (("mixed", head_vol), ("fp32_plain", plain_vol)):
+        vols = [("mixed", head_vol), ("fp32_plain", plain_vol)] + \
+            [(k, v) for k, v in variant_vols.items()]
+        for name, v in vols:
             if v is None:
                 continue
-            d = (v[idx] - ref) * scan.hu_per_unit()
-            out[f"hu_vs_fp64_oracle_{name}"] = {
-                "rmse": round(float(np.sqrt(np.mean(d * d))), 4),
-                "max_abs": round(float(np.max(np.abs(d))), 3),
-                "rel_rmse": float(np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref)),
-                "sample": f"block 0 slices {idx}, all voxels"}
+            s = hu_stats(v[idx], ref, np, scan)
+            s["rel_rmse"] = float(np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref))
+            s["sample"] = f"block 0 slices {idx}, all voxels"
+            out[f"hu_vs_fp64_oracle_{name}"] = s
     ctx.close()
     print(json.dumps(out), flush=True)
 

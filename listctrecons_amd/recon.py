@@ -162,6 +162,11 @@ class Context:
     def synchronize(self) -> None:
         check(lib().lct_synchronize(self.h))
 
+    def set_stream(self, stream: int) -> None:
+        """Run this context's work on a HIP stream handle (e.g. a torch
+        stream's `cuda_stream`); 0 is the default stream."""
+        check(lib().lct_ctx_set_stream(self.h, C.c_void_p(stream)))
+
     def set_profiling(self, on: bool) -> None:
         check(lib().lct_set_profiling(self.h, 1 if on else 0))
 
