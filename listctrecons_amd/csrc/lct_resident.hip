@@ -215,7 +215,11 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // multiple of 8, which would put them in a quarter of the banks).  LP > 0:
 // lp known at compile time, so the two channels of a bilinear tap are read
 // with immediate LDS offsets from one address.
-template <typename T, int NPRE, int LP>
+//
+// G (LCT_INTERP_GATHER): the same arithmetic with every tap gathered from Q
+// in HBM through the L1 / L2 caches instead: no staging, no barriers; the
+// zero guard slots of Q give the reference's border behaviour.
+template <typename T, int NPRE, int LP, bool G = false>
 __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 {
     extern __shared__ float lds[];
@@ -312,21 +316,25 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         bc2 = cosq[jj + A.h];
         bs2 = sinq[jj + A.h];
         bzm = int(A.zmask[jj - jlo]);
-        int w1, w2;
-        win_of(bc, bs, w1, w2);
-        bwin = w1 | (w2 << 16);
+        if constexpr (!G) {
+            int w1, w2;
+            win_of(bc, bs, w1, w2);
+            bwin = w1 | (w2 << 16);
+        } else {
+            bwin = 0;
+        }
     };
     auto lanef = [](float v, int l) {
         return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
     };
     int b0 = jlo;
     batch(b0);
-    {
+    if constexpr (!G) {
         const int w = __builtin_amdgcn_readlane(bwin, 0);
         fetch(jlo, w & 0xffff, w >> 16);
+        commit(lds);
+        __syncthreads();
     }
-    commit(lds);
-    __syncthreads();
     int cur = 0;
     for (int j = jlo; j < jhi; j++) {
         const int l = j - b0;
@@ -339,9 +347,11 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
             b0 += 64;
             batch(b0);
         }
-        // stage pair j + 1 (the last pair is staged twice, into the idle buffer)
-        const int wn = __builtin_amdgcn_readlane(bwin, j + 1 - b0);
-        fetch(min(j + 1, jhi - 1), wn & 0xffff, wn >> 16);
+        if constexpr (!G) {
+            // stage pair j + 1 (the last pair is staged twice, into the idle buffer)
+            const int wn = __builtin_amdgcn_readlane(bwin, j + 1 - b0);
+            fetch(min(j + 1, jhi - 1), wn & 0xffff, wn >> 16);
+        }
         const float *buf = lds + cur * stage;
         // z-independent terms of the pair (Backprojector.cu:53-62)
         const float phat = -xc * c - yc * s;
@@ -358,6 +368,9 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const int l2 = min(max(int(fp2) + 1 - s2, 0), W - 2);
         const float *b1 = buf + __mul24(l1, lp);
         const float *b2 = buf + lds_view + __mul24(l2, lp);
+        // gather: channel slot = channel + 1 (slots 0, ns + 1 and ns + 2 are zero)
+        const T *g1 = Q + size_t(qoff + j) * A.view_stride + size_t(int(fp1) + 1) * rp;
+        const T *g2 = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(int(fp2) + 1) * rp;
         const float root = __builtin_amdgcn_sqrtf(A.dso2 - phat * phat);
         const float lhat = root + xc * s - yc * c;
         const float lhat2 = root + xc * s2v - yc * c2;
@@ -387,10 +400,30 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
                 const float r2 = __builtin_amdgcn_fmed3f(qh2 + row0, lo_r, hi_r);
                 const int i1 = int(r1), i2 = int(r2);
                 const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
-                const float *p1 = b1 + i1;
-                const float *p2 = b2 + i2;
-                const float a00 = p1[0], a01 = p1[1], a10 = p1[lp], a11 = p1[lp + 1];
-                const float c00 = p2[0], c01 = p2[1], c10 = p2[lp], c11 = p2[lp + 1];
+                float a00, a01, a10, a11, c00, c01, c10, c11;
+                if constexpr (G) {
+                    const T *p1 = g1 + i1;
+                    const T *p2 = g2 + i2;
+                    a00 = Cvt<T>::from(p1[0]);
+                    a01 = Cvt<T>::from(p1[1]);
+                    a10 = Cvt<T>::from(p1[rp]);
+                    a11 = Cvt<T>::from(p1[rp + 1]);
+                    c00 = Cvt<T>::from(p2[0]);
+                    c01 = Cvt<T>::from(p2[1]);
+                    c10 = Cvt<T>::from(p2[rp]);
+                    c11 = Cvt<T>::from(p2[rp + 1]);
+                } else {
+                    const float *p1 = b1 + i1;
+                    const float *p2 = b2 + i2;
+                    a00 = p1[0];
+                    a01 = p1[1];
+                    a10 = p1[lp];
+                    a11 = p1[lp + 1];
+                    c00 = p2[0];
+                    c01 = p2[1];
+                    c10 = p2[lp];
+                    c11 = p2[lp + 1];
+                }
                 const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
                 const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
                 const float t1 = fmaf(ar1, ha - la, la);
@@ -398,91 +431,10 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
                 acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
             }
         }
-        commit(lds + (cur ^ 1) * stage);
-        __syncthreads();
-        cur ^= 1;
-    }
-    if (ix < A.nx && iy < A.ny) {
-        float *o = A.vol + (size_t(slice0) * A.ny + iy) * A.nx + ix;
-        const size_t plane = size_t(A.nx) * A.ny;
-#pragma unroll
-        for (int z = 0; z < 32; z++) o[size_t(z) * plane] = acc[z] * A.scale;
-    }
-}
-
-// The same pair-folded backprojection with every bilinear tap gathered from
-// HBM through the vector L1 / L2 caches (no LDS staging, no barriers).  The
-// zero guard slots of Q give the reference's border behaviour.
-template <typename T>
-__global__ __launch_bounds__(256) void bp_fold_gather_kernel(FoldBpArgs A)
-{
-    int wg = blockIdx.x;
-    if (A.xcd_remap) wg = (wg & 7) * (A.n_wg >> 3) + (wg >> 3);
-    const int tiles = A.n_tiles_x * A.n_tiles_y;
-    const int blk = wg / tiles;
-    const int t = wg - blk * tiles;
-    const int tx0 = (t % A.n_tiles_x) * kTile, ty0 = (t / A.n_tiles_x) * kTile;
-    const T *Q = static_cast<const T *>(A.q);
-    const int rp = A.rp;
-    const int ix = tx0 + int(threadIdx.x & (kTile - 1));
-    const int iy = ty0 + int(threadIdx.x / kTile);
-    const int *st = A.slab_tab + blk * 34;
-    const int qoff = st[0], slice0 = st[1];
-    int am[32];
-#pragma unroll
-    for (int z = 0; z < 32; z++) am[z] = st[2 + z];
-    const float xc = (float(ix) - A.wx) * A.dx;
-    const float yc = (float(iy) - A.wy) * A.dy;
-    float acc[32];
-#pragma unroll
-    for (int z = 0; z < 32; z++) acc[z] = 0.0f;
-    const int jlo = am[0] - A.h, jhi = am[31];
-    for (int j = jlo; j < jhi; j++) {
-        const float c = A.cosb[qoff + j], s = A.sinb[qoff + j];
-        const float c2 = A.cosb[qoff + j + A.h], s2v = A.sinb[qoff + j + A.h];
-        const float phat = -xc * c - yc * s;
-        const float theta = asinf(phat * A.inv_dso);
-        const float delta = theta * A.dso_ds;
-        const float p1 = fminf(fmaxf(A.ws + delta, -1.0f), A.ns_f);
-        const float p2 = fminf(fmaxf(A.ws - delta, -1.0f), A.ns_f);
-        const float fp1 = floorf(p1), fp2 = floorf(p2);
-        const float ap1 = p1 - fp1, ap2 = p2 - fp2;
-        // channel slot = channel + 1 (slots 0 and ns+1, ns+2 are zero)
-        const T *b1 = Q + size_t(qoff + j) * A.view_stride + size_t(int(fp1) + 1) * rp;
-        const T *b2 = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(int(fp2) + 1) * rp;
-        const float root = sqrtf(A.dso2 - phat * phat);
-        const float lhat = root + xc * s - yc * c;
-        const float lhat2 = root + xc * s2v - yc * c2;
-        const float tz = theta * A.inv_orbit;
-        const float zr = A.z0 + (float(j) + tz) * A.dzv;
-        const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
-        const float k1 = A.kc / lhat, k2 = A.kc / lhat2;
-        const float o1 = -zr * k1, o2 = -zr2 * k2;
-#pragma unroll
-        for (int z = 0; z < 32; z++) {
-            if (am[z] > j && am[z] <= j + A.h) {
-                const float zc = (float(z) - A.wz) * A.dz;
-                const float qh = fmaf(zc, k1, o1);
-                const float qh2 = fmaf(zc, k2, o2);
-                const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
-                const float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
-                const float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
-                const float w2 = out1 ? (out2 ? 0.5f : 1.0f) : (out2 ? 0.0f : 1.0f - wb);
-                const float r1 = fminf(fmaxf(qh + A.wt, -1.0f), A.nt_f);
-                const float r2 = fminf(fmaxf(qh2 + A.wt, -1.0f), A.nt_f);
-                const float fr1 = floorf(r1), fr2 = floorf(r2);
-                const float ar1 = r1 - fr1, ar2 = r2 - fr2;
-                const int i1 = int(fr1) + kRowPad, i2 = int(fr2) + kRowPad;
-                const float a00 = Cvt<T>::from(b1[i1]), a01 = Cvt<T>::from(b1[i1 + 1]);
-                const float a10 = Cvt<T>::from(b1[rp + i1]), a11 = Cvt<T>::from(b1[rp + i1 + 1]);
-                const float c00 = Cvt<T>::from(b2[i2]), c01 = Cvt<T>::from(b2[i2 + 1]);
-                const float c10 = Cvt<T>::from(b2[rp + i2]), c11 = Cvt<T>::from(b2[rp + i2 + 1]);
-                const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
-                const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
-                const float t1 = fmaf(ar1, ha - la, la);
-                const float t2 = fmaf(ar2, hc - lc, lc);
-                acc[z] = fmaf(w1, t1, fmaf(w2, t2, acc[z]));
-            }
+        if constexpr (!G) {
+            commit(lds + (cur ^ 1) * stage);
+            __syncthreads();
+            cur ^= 1;
         }
     }
     if (ix < A.nx && iy < A.ny) {
@@ -498,7 +450,10 @@ static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
 {
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
-    hipLaunchKernelGGL(bp_fold_gather_kernel<T>, dim3(unsigned(a.n_wg)), dim3(256), 0, ctx->stream, a);
+    a.lp = a.rp + 1;
+    a.W = 2;  // no LDS window
+    hipLaunchKernelGGL((bp_fold_kernel<T, 1, 0, true>), dim3(unsigned(a.n_wg)), dim3(256), 0,
+                       ctx->stream, a);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
