@@ -219,6 +219,13 @@ int lct_reconstruct_slabs(lct_ctx *ctx, const lct_plan *plan, const lct_recon_op
                           const float *h_angles, const float *h_couch, const float *d_fan,
                           long fan_view0, long fan_views, int block_begin, int block_end,
                           float *d_vol);
+/* The filtered views [n_angle][nt][ns] (f32) of the last block the last
+ * lct_reconstruct / lct_reconstruct_slabs call on this context reconstructed:
+ * what Filter::filter leaves in CTDataCPU::rebin (CUDA/Filter.cu:69) and
+ * RawImageWriter writes to rebin.raw (Src/ImageWriter.cpp:27-40).  With
+ * f16 / bf16 storage they carry that rounding.  Valid until the next
+ * reconstruct call on the context. */
+int lct_last_block_filtered(lct_ctx *ctx, float *d_rows);
 int lct_get_timings(lct_ctx *ctx, lct_timings *out);
 /* Enables HIP-event timing of every stage launch (off by default). */
 int lct_set_profiling(lct_ctx *ctx, int enable);

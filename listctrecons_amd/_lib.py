@@ -133,6 +133,7 @@ SIGNATURES = {
     "lct_reconstruct": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long, _vp]),
     "lct_reconstruct_slabs": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long,
                                         C.c_long, C.c_int, C.c_int, _vp]),
+    "lct_last_block_filtered": (C.c_int, [_vp, _vp]),
     "lct_get_timings": (C.c_int, [_vp, _P(Timings)]),
     "lct_set_profiling": (C.c_int, [_vp, C.c_int]),
     "lct_project_ellipsoids": (C.c_int, [_vp, _P(CTGeom), _fp, _fp, C.c_long, C.c_long, _dp,

@@ -206,6 +206,17 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
             std::exit(3);
         }
     }
+    // rebin.raw: the last block's filtered views, as the per-block path leaves
+    // them in CTDataCPU::rebin (CUDA/Filter.cu:69)
+    if (start < plan.n_blocks) {
+        const size_t n_rb = size_t(plan.ctg.n_angle) * view;
+        GPUMemoryCtrl<float> rb;
+        if (!rb.allocate_memory(n_rb) ||
+            lct_last_block_filtered(lct_default_ctx(), rb.get_pointer()) != LCT_OK ||
+            lct_synchronize(lct_default_ctx()) != LCT_OK ||
+            !rb.copy_to_CPU(mr->data_cpu()->rebin_(), n_rb))
+            fail(lct_last_error());
+    }
     if (checkpoint) {
         std::fclose(out);
         std::remove(ck_path.c_str());

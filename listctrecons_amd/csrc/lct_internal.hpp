@@ -29,6 +29,16 @@ struct lct_ctx {
 
     hipEvent_t ev[8] = {};
     bool events_ready = false;
+
+    // Where the last reconstruct call left its last block's filtered views
+    // (scratch slots A / B, untouched until the next reconstruct call):
+    // layout 1: rows [view][nt][ns] f32; layout 2: Q [view][cp][rp] of
+    // `storage`; 0: nothing.  The block's views start at view0.
+    struct LastBlock {
+        int layout = 0, storage = 0, n_angle = 0, nt = 0, ns = 0, cp = 0, rp = 0;
+        long view0 = 0;
+        const void *ptr = nullptr;
+    } last;
 };
 
 namespace lct {

@@ -103,6 +103,9 @@ static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_reco
         LCT_CHECK(clk.stop(&t.backproject_ms));
         t.n_backproject_launches++;
     }
+    // the last block's filtered views stay in `par` (CTDataCPU::rebin of the
+    // reference after Filter.cu:69)
+    ctx->last = lct_ctx::LastBlock{1, LCT_F32, g.n_angle, g.nt, g.ns, 0, 0, 0, par};
     return LCT_OK;
 }
 
@@ -133,6 +136,7 @@ static int reconstruct_range(lct_ctx *ctx, const lct_plan *plan, const lct_recon
     p.data_offset = int(long(plan->data_offset) - fan_view0);
     LCT_CHECK_HIP(hipSetDevice(ctx->device));
     ctx->timings = lct_timings{};
+    ctx->last = lct_ctx::LastBlock{};
     hipEvent_t t0 = ctx->ev[4], t1 = ctx->ev[5];
     if (ctx->profiling) LCT_CHECK_HIP(hipEventRecord(t0, ctx->stream));
     int rc;

@@ -638,6 +638,8 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
             LCT_CHECK(toc(&t.backproject_ms));
             t.n_backproject_launches++;
         }
+        ctx->last = lct_ctx::LastBlock{1, LCT_F32, g.n_angle, g.nt, g.ns, 0, 0,
+                                       long(R.bps[nb - 1]) + g.add_projections - R.g0, P};
         return LCT_OK;
     }
 
@@ -742,12 +744,57 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, a, nb));
     LCT_CHECK(toc(&t.backproject_ms));
     t.n_backproject_launches++;
+    ctx->last = lct_ctx::LastBlock{2, o.storage, g.n_angle, g.nt, g.ns, cp, rp,
+                                   long(R.bps[nb - 1]) + g.add_projections - R.g0, d_q};
     return LCT_OK;
+}
+
+// Q[view0 + v][c + 1][r + kRowPad] -> rows[v][r][c] (f32).
+template <typename T>
+__global__ __launch_bounds__(256) void q_to_rows_kernel(const T *__restrict__ q, long view0,
+                                                        int cp, int rp, int ns, int nt,
+                                                        float *__restrict__ rows)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    const long v = blockIdx.z;
+    if (c >= ns) return;
+    rows[(size_t(v) * nt + r) * ns + c] =
+        Cvt<T>::from(q[(size_t(view0 + v) * cp + c + 1) * rp + r + kRowPad]);
 }
 
 }  // namespace lct
 
 using namespace lct;
+
+extern "C" int lct_last_block_filtered(lct_ctx *ctx, float *d_rows)
+{
+    if (!ctx || !d_rows) return set_error(LCT_ERR_INVALID, "lct_last_block_filtered: NULL argument");
+    const lct_ctx::LastBlock &L = ctx->last;
+    if (L.layout == 0)
+        return set_error(LCT_ERR_INVALID, "lct_last_block_filtered: no reconstruction on this context");
+    LCT_CHECK_HIP(hipSetDevice(ctx->device));
+    const size_t view = size_t(L.nt) * L.ns;
+    if (L.layout == 1) {
+        LCT_CHECK_HIP(hipMemcpyAsync(d_rows, static_cast<const float *>(L.ptr) + L.view0 * view,
+                                     view * L.n_angle * sizeof(float), hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+        return LCT_OK;
+    }
+    const dim3 grid(unsigned((L.ns + 255) / 256), unsigned(L.nt), unsigned(L.n_angle));
+    if (L.storage == LCT_F32)
+        hipLaunchKernelGGL(q_to_rows_kernel<float>, grid, dim3(256), 0, ctx->stream,
+                           static_cast<const float *>(L.ptr), L.view0, L.cp, L.rp, L.ns, L.nt, d_rows);
+    else if (L.storage == LCT_F16)
+        hipLaunchKernelGGL(q_to_rows_kernel<__half>, grid, dim3(256), 0, ctx->stream,
+                           static_cast<const __half *>(L.ptr), L.view0, L.cp, L.rp, L.ns, L.nt, d_rows);
+    else
+        hipLaunchKernelGGL(q_to_rows_kernel<__hip_bfloat16>, grid, dim3(256), 0, ctx->stream,
+                           static_cast<const __hip_bfloat16 *>(L.ptr), L.view0, L.cp, L.rp, L.ns,
+                           L.nt, d_rows);
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
 
 extern "C" int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int ns, int nt,
                                        const float *h_taps, const float *d_rows, long views,

@@ -218,6 +218,16 @@ class Context:
                                     _dev(fan), views, _dev(vol)))
         return vol
 
+    def last_block_filtered(self, plan: _lib.Plan, out=None):
+        """Filtered views [n_angle][nt][ns] of the last reconstructed block
+        (what the reference writes to rebin.raw)."""
+        import torch
+        g = plan.ctg
+        if out is None:
+            out = torch.empty((g.n_angle, g.nt, g.ns), dtype=torch.float32, device="cuda")
+        check(lib().lct_last_block_filtered(self.h, _dev(out)))
+        return out
+
     def reconstruct_slabs(self, plan: _lib.Plan, opts: Options, angles: np.ndarray,
                           couch: np.ndarray, fan, fan_view0: int, block_begin: int,
                           block_end: int, vol=None):

@@ -93,6 +93,26 @@ def test_variants(ctx, preset, name, wkw, okw):
     assert rel_rmse(got, ref) <= 1e-4
 
 
+@pytest.mark.parametrize("name,wkw,preset", [
+    ("tiny", {"readings": 1400, "pitch": 1.7}, "plain"),
+    ("tiny", {"readings": 1400, "pitch": 1.7}, "fast_f32"),
+    ("tiny", {"n_proj_turn": 97, "readings": 1200}, "fast_f32"),  # resident, not folded
+])
+def test_last_block_filtered_views(ctx, name, wkw, preset):
+    """lct_last_block_filtered == the oracle's rebinned + filtered window of
+    the last block (the reference's rebin.raw)."""
+    w = scan.workload(name, **wkw)
+    pl, a, c = oracle_plan(w)
+    fan = cpu_fan(w)
+    gpu_recon(ctx, w, fan, preset)
+    got = ctx.last_block_filtered(recon.plan_helical(w.prm(), a, c)).cpu().numpy()
+    g = pl.ctg
+    s, _, _, _ = O.block_info(pl, c, pl.n_blocks - 1)
+    first = s + pl.data_offset
+    q = O.filter_rows(O.ramp_taps(g.ns, g.ds, w.window), O.rebin(g, fan[first:first + g.n_angle_add]))
+    assert rel_rmse(got, q) < 2e-6
+
+
 def test_missing_views_raise(ctx):
     w = scan.workload("tiny")
     a, c = w.angles_couch()
