@@ -234,8 +234,9 @@ def main():
                     help="run one device copy of this many GiB first (PMC scale reference)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the helical FDK path shards by "
-                                   "z-slab (see DESIGN.md) but only the single-GPU kernel is "
+        print(json.dumps({"error": "runs on one GPU: the helical FDK path would shard by "
+                                   "32-slice blocks (each GPU a block range and the views it "
+                                   "reads, no collectives), but only the single-GPU path is "
                                    "built and measured; run with --gpus 1"}))
         sys.exit(2)
 
