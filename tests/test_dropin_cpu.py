@@ -2,6 +2,7 @@
 reference's own Src/FileParams.cpp (built into oracle/_ref by oracle/ref.mk
 when the reference sources are present), the CLI's planner against the
 oracle, and the reference's exit(1) paths."""
+import json
 import os
 import subprocess
 
@@ -50,6 +51,22 @@ def test_prm_parser_matches_reference(files, tmp_path):
         ours = run(CLI, "--prm", p)
         assert ref.returncode == 0 and ours.returncode == 0, name
         assert ours.stdout == ref.stdout, name
+
+
+GOLDEN = json.load(open(os.path.join(ROOT, "tests", "golden", "prm_parser.json")))
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_prm_parser_matches_reference_vectors(name, tmp_path):
+    """The reference's own FileParams.cpp output for 12 .prm files
+    (tests/golden/make_prm_golden.py), replayed without the reference."""
+    v = GOLDEN[name]
+    p = tmp_path / (name + ".prm")
+    with open(p, "w", newline="") as f:
+        f.write(v["prm"])
+    ours = run(CLI, "--prm", str(p))
+    assert ours.returncode == v["returncode"]
+    assert ours.stdout == v["stdout"]
 
 
 @pytest.mark.skipif(not os.path.exists(REF_PARSER), reason="reference parser not built")
