@@ -59,5 +59,8 @@ def test_training_lowers_rmse_on_held_out_scans(ctx, w, cases):
         gains.append(1 - e_cor / e_fdk)
         print(f"held-out rmse fdk {e_fdk:.3e} corrected {e_cor:.3e} gain {gains[-1]:.1%}; "
               f"reprojection fdk {r_fdk:.4f} corrected {r_cor:.4f}", flush=True)
-    assert min(gains) > 0.10
+        assert r_cor < r_fdk  # the reprojection gate keeps the corrected volume
+    # the target of the learned row: at least 30 % lower RMSE to the phantom
+    # (measured: 46 % and 48 % on these two held-out scans after 300 steps)
+    assert min(gains) >= 0.30
     net.close()
