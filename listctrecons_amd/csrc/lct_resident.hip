@@ -226,8 +226,7 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     constexpr int CE = Chunk<T>::kElems;
     const int W = A.W, rp = A.rp, lp = LP > 0 ? LP : A.lp;
     const int view_elems = W * rp;          // one view's window in Q
-    const int lds_view = W * lp;            // and in LDS
-    const int stage = 2 * lds_view;         // views j and j+h
+    const int lds_view = W * lp;            // and in LDS (views j, j+h: one buffer of 2 W lp)
     // 16-byte chunks: view j's at chunk indices [0, n_vc), view j+h's at
     // [pad, pad + n_vc) with pad a multiple of 64, so each wave's chunks of a
     // slot belong to one view and share a scalar base address
@@ -335,7 +334,6 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         commit(lds);
         __syncthreads();
     }
-    int cur = 0;
     for (int j = jlo; j < jhi; j++) {
         const int l = j - b0;
         const float c = lanef(bc, l), s = lanef(bs, l);
@@ -348,11 +346,11 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
             batch(b0);
         }
         if constexpr (!G) {
-            // stage pair j + 1 (the last pair is staged twice, into the idle buffer)
+            // prefetch pair j + 1 into registers (the last pair is fetched twice)
             const int wn = __builtin_amdgcn_readlane(bwin, j + 1 - b0);
             fetch(min(j + 1, jhi - 1), wn & 0xffff, wn >> 16);
         }
-        const float *buf = lds + cur * stage;
+        const float *buf = lds;
         // z-independent terms of the pair (Backprojector.cu:53-62)
         const float phat = -xc * c - yc * s;
         const float u = phat * A.inv_dso;
@@ -378,7 +376,10 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const float zr = A.z0 + (float(j) + tz) * A.dzv;
         const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
         const float k1 = A.kc * __builtin_amdgcn_rcpf(lhat), k2 = A.kc * __builtin_amdgcn_rcpf(lhat2);
-        const float o1 = -zr * k1, o2 = -zr2 * k2;
+        // detector row offsets qh(z) = (zc(z) - zr) k1 as z K1 + O1 with zc(z) = (z - wz) dz:
+        // the slice index is a literal, so no per-slice coordinate is held in a register
+        const float K1 = k1 * A.dz, K2 = k2 * A.dz;
+        const float O1 = fmaf(-A.wz, K1, -zr * k1), O2 = fmaf(-A.wz, K2, -zr2 * k2);
         // row coordinate in slot units (row + kRowPad), clamped to the guard
         // slots [kRowPad - 1, nt + kRowPad]: positive, so truncation is floor
         const float row0 = A.wt + float(kRowPad);
@@ -386,9 +387,8 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 #pragma unroll
         for (int z = 0; z < 32; z++) {
             if (zm & (1u << z)) {  // pair inside this slice's window
-                const float zc = (float(z) - A.wz) * A.dz;
-                const float qh = fmaf(zc, k1, o1);
-                const float qh2 = fmaf(zc, k2, o2);
+                const float qh = fmaf(float(z), K1, O1);
+                const float qh2 = fmaf(float(z), K2, O2);
                 const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
                 float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
                 // keeps wb computed on every lane, so the selects below stay
@@ -432,9 +432,12 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
             }
         }
         if constexpr (!G) {
-            commit(lds + (cur ^ 1) * stage);
+            // one buffer: every wave is done with pair j before pair j + 1
+            // overwrites it (half the LDS of double buffering, so five
+            // workgroups fit a CU instead of four)
             __syncthreads();
-            cur ^= 1;
+            commit(lds);
+            __syncthreads();
         }
     }
     if (ix < A.nx && iy < A.ny) {
@@ -468,7 +471,7 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     // chunk slots: view j's chunks, padded to a multiple of 64, then view j+h's
     const int n_vc = a.W * a.rp / CE;
     const int chunks = ((n_vc + 63) & ~63) + n_vc;
-    const size_t shm = size_t(2) * 2 * a.W * a.lp * sizeof(float);
+    const size_t shm = size_t(2) * a.W * a.lp * sizeof(float);
     if (shm > 160 * 1024)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
     const int npre = (chunks + 255) / 256;
