@@ -262,6 +262,17 @@ int lct_forward_project(lct_ctx *ctx, const lct_ct_geom *g, const float *h_angle
                         const lct_img_geom *vg, float vol_z0, int vol_nz, float step,
                         float *d_out);
 
+/* Adjoint (transpose) of lct_forward_project over the same rays, samples and
+ * trilinear weights: d_vol [vol_nz][ny][nx] = A^T d_proj, d_proj
+ * [n_views][nt][ns].  d_vol is overwritten.  Each sample's weighted value is
+ * added with float atomics, so the summation order, and the last bits, vary
+ * from run to run.  Not part of the reference: with lct_forward_project it
+ * forms a matched projector pair, <A x, y> = <x, A^T y>. */
+int lct_forward_project_adjoint(lct_ctx *ctx, const lct_ct_geom *g, const float *h_angles,
+                                const float *h_couch, long view0, long n_views,
+                                const float *d_proj, const lct_img_geom *vg, float vol_z0,
+                                int vol_nz, float step, float *d_vol);
+
 /* ------------------------------------------- learned artifact correction
  * Not part of the reference.  A 3D residual CNN, y = x + net(x), net = six
  * 3x3x3 convolutions 1 -> 32 -> 32 -> 32 -> 32 -> 32 -> 1 with ReLU after the

@@ -22,6 +22,14 @@ def _fptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _check_f32(t, shape, name: str) -> None:
+    """A float32 tensor of exactly `shape` (what the kernel will index)."""
+    import torch
+    if t.dtype != torch.float32 or tuple(t.shape) != tuple(int(x) for x in shape):
+        raise ValueError(f"{name}: expected float32 of shape {tuple(shape)}, "
+                         f"got {t.dtype} {tuple(t.shape)}")
+
+
 def _dev(t) -> C.c_void_p:
     import torch
     if not isinstance(t, torch.Tensor) or not t.is_cuda or not t.is_contiguous():
@@ -268,9 +276,27 @@ class Context:
         c = np.ascontiguousarray(couch, np.float32)
         if n_views is None:
             n_views = a.size - view0
+        _check_f32(vol, (vol.shape[0], ig.ny, ig.nx), "vol")
+        _check_f32(out, (n_views, g.nt, g.ns), "out")
+        if view0 < 0 or view0 + n_views > min(a.size, c.size):
+            raise ValueError("views outside the angle / couch tables")
         check(lib().lct_forward_project(self.h, C.byref(g), _fptr(a), _fptr(c), view0, n_views,
                                         _dev(vol), C.byref(ig), C.c_float(z0), vol.shape[0],
                                         C.c_float(step), _dev(out)))
+
+    def forward_project_adjoint(self, g: _lib.CTGeom, angles, couch, proj, ig: _lib.ImgGeom,
+                                z0: float, out, view0: int = 0, step: float = 0.25) -> None:
+        """Transpose of forward_project: proj [views][nt][ns] -> out [nz][ny][nx]."""
+        a = np.ascontiguousarray(angles, np.float32)
+        c = np.ascontiguousarray(couch, np.float32)
+        _check_f32(proj, (proj.shape[0], g.nt, g.ns), "proj")
+        _check_f32(out, (out.shape[0], ig.ny, ig.nx), "out")
+        if view0 < 0 or view0 + proj.shape[0] > min(a.size, c.size):
+            raise ValueError("views outside the angle / couch tables")
+        check(lib().lct_forward_project_adjoint(self.h, C.byref(g), _fptr(a), _fptr(c), view0,
+                                                proj.shape[0], _dev(proj), C.byref(ig),
+                                                C.c_float(z0), out.shape[0], C.c_float(step),
+                                                _dev(out)))
 
     def voxelize(self, ig: _lib.ImgGeom, z0: float, nz: int, ell: np.ndarray, out,
                  supersample: int = 2) -> None:

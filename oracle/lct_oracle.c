@@ -693,39 +693,81 @@ static double trilinear(const double *vol, int nx, int ny, int nz, double fx, do
     return acc;
 }
 
+/* Adjoint of trilinear(): adds val times each of the eight weights to the
+ * voxels that trilinear() reads (the same zero-border rule). */
+static void trilinear_scatter(double *vol, int nx, int ny, int nz, double fx, double fy,
+                              double fz, double val)
+{
+    const double x0f = floor(fx), y0f = floor(fy), z0f = floor(fz);
+    const int x0 = (int)x0f, y0 = (int)y0f, z0 = (int)z0f;
+    const double ax = fx - x0f, ay = fy - y0f, az = fz - z0f;
+    for (int k = 0; k < 2; k++) {
+        const int zz = z0 + k;
+        if (zz < 0 || zz >= nz) continue;
+        for (int j = 0; j < 2; j++) {
+            const int yy = y0 + j;
+            if (yy < 0 || yy >= ny) continue;
+            for (int i = 0; i < 2; i++) {
+                const int xx = x0 + i;
+                if (xx < 0 || xx >= nx) continue;
+                const double w = (k ? az : 1.0 - az) * (j ? ay : 1.0 - ay) * (i ? ax : 1.0 - ax);
+#pragma omp atomic
+                vol[((size_t)zz * ny + yy) * nx + xx] += w * val;
+            }
+        }
+    }
+}
+
+/* The ray of view (b, sz), row it, channel is (source S, unit direction d)
+ * and its chord [t0, t1] through the box of voxel centres lo..hi; returns 0
+ * when the ray misses the box. */
+static int ray_chord(const lco_ct_geom *g, double b, double sz, int it, int is,
+                     const double lo[3], const double hi[3], double S[3], double d[3],
+                     double *t0_out, double *t1_out)
+{
+    const double dso = g->dso;
+    const double slope = ((double)it - (double)g->wt) * (double)g->dt / dso;
+    const double inv = 1.0 / sqrt(1.0 + slope * slope);
+    const double a = ((double)is - (double)g->ws) * (double)g->ds / dso;
+    S[0] = -dso * sin(b);
+    S[1] = dso * cos(b);
+    S[2] = sz;
+    d[0] = sin(b - a) * inv;
+    d[1] = -cos(b - a) * inv;
+    d[2] = slope * inv;
+    double t0 = -1e300, t1 = 1e300;
+    for (int k = 0; k < 3; k++) {
+        if (fabs(d[k]) < 1e-300) {
+            if (S[k] < lo[k] || S[k] > hi[k]) t1 = -1e300;
+            continue;
+        }
+        double ta = (lo[k] - S[k]) / d[k], tb = (hi[k] - S[k]) / d[k];
+        if (ta > tb) { const double tmp = ta; ta = tb; tb = tmp; }
+        if (ta > t0) t0 = ta;
+        if (tb < t1) t1 = tb;
+    }
+    *t0_out = t0;
+    *t1_out = t1;
+    return t1 > t0;
+}
+
 void lco_forward_project(const lco_ct_geom *g, const float *angles, const float *couch,
                          long view0, long n_views, const double *vol, const lco_img_geom *ig,
                          double z0, int nz, double step, double *out)
 {
     const int ns = g->ns, nt = g->nt, nx = ig->nx, ny = ig->ny;
-    const double dso = g->dso, dx = ig->dx, dy = ig->dy, dz = ig->dz;
+    const double dx = ig->dx, dy = ig->dy, dz = ig->dz;
     const double wx = ig->wx, wy = ig->wy;
     const double lo[3] = {-wx * dx, -wy * dy, z0};
     const double hi[3] = {(nx - 1 - wx) * dx, (ny - 1 - wy) * dy, z0 + (nz - 1) * dz};
 #pragma omp parallel for schedule(dynamic, 1)
     for (long v = 0; v < n_views; v++) {
         const double b = (double)angles[view0 + v];
-        const double S[3] = {-dso * sin(b), dso * cos(b), (double)couch[view0 + v]};
         for (int it = 0; it < nt; it++) {
-            const double slope = ((double)it - (double)g->wt) * (double)g->dt / dso;
-            const double inv = 1.0 / sqrt(1.0 + slope * slope);
             for (int is = 0; is < ns; is++) {
-                const double a = ((double)is - (double)g->ws) * (double)g->ds / dso;
-                const double d[3] = {sin(b - a) * inv, -cos(b - a) * inv, slope * inv};
-                /* chord of the ray through the box of voxel centres */
-                double t0 = -1e300, t1 = 1e300;
-                for (int k = 0; k < 3; k++) {
-                    if (fabs(d[k]) < 1e-300) {
-                        if (S[k] < lo[k] || S[k] > hi[k]) t1 = -1e300;
-                        continue;
-                    }
-                    double ta = (lo[k] - S[k]) / d[k], tb = (hi[k] - S[k]) / d[k];
-                    if (ta > tb) { const double tmp = ta; ta = tb; tb = tmp; }
-                    if (ta > t0) t0 = ta;
-                    if (tb < t1) t1 = tb;
-                }
+                double S[3], d[3], t0, t1;
                 double sum = 0.0;
-                if (t1 > t0) {
+                if (ray_chord(g, b, (double)couch[view0 + v], it, is, lo, hi, S, d, &t0, &t1)) {
                     const long n = (long)ceil((t1 - t0) / step);
                     const double h = (t1 - t0) / (double)n;
                     for (long k = 0; k < n; k++) {
@@ -736,6 +778,39 @@ void lco_forward_project(const lco_ct_geom *g, const float *angles, const float 
                     sum *= h;
                 }
                 out[((size_t)v * nt + it) * ns + is] = sum;
+            }
+        }
+    }
+}
+
+void lco_forward_project_adjoint(const lco_ct_geom *g, const float *angles, const float *couch,
+                                 long view0, long n_views, const double *proj,
+                                 const lco_img_geom *ig, double z0, int nz, double step,
+                                 double *vol)
+{
+    const int ns = g->ns, nt = g->nt, nx = ig->nx, ny = ig->ny;
+    const double dx = ig->dx, dy = ig->dy, dz = ig->dz;
+    const double wx = ig->wx, wy = ig->wy;
+    const double lo[3] = {-wx * dx, -wy * dy, z0};
+    const double hi[3] = {(nx - 1 - wx) * dx, (ny - 1 - wy) * dy, z0 + (nz - 1) * dz};
+    memset(vol, 0, sizeof(double) * (size_t)nz * ny * nx);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long v = 0; v < n_views; v++) {
+        const double b = (double)angles[view0 + v];
+        for (int it = 0; it < nt; it++) {
+            for (int is = 0; is < ns; is++) {
+                double S[3], d[3], t0, t1;
+                if (!ray_chord(g, b, (double)couch[view0 + v], it, is, lo, hi, S, d, &t0, &t1))
+                    continue;
+                const long n = (long)ceil((t1 - t0) / step);
+                const double h = (t1 - t0) / (double)n;
+                const double val = h * proj[((size_t)v * nt + it) * ns + is];
+                for (long k = 0; k < n; k++) {
+                    const double t = t0 + ((double)k + 0.5) * h;
+                    trilinear_scatter(vol, nx, ny, nz, (S[0] + t * d[0]) / dx + wx,
+                                      (S[1] + t * d[1]) / dy + wy, (S[2] + t * d[2] - z0) / dz,
+                                      val);
+                }
             }
         }
     }

@@ -154,6 +154,13 @@ void lco_forward_project(const lco_ct_geom *g, const float *angles, const float 
                          long view0, long n_views, const double *vol, const lco_img_geom *ig,
                          double z0, int nz, double step, double *out);
 
+/* Adjoint (transpose) of lco_forward_project: vol = A^T proj for the same rays,
+ * samples and trilinear weights.  vol [nz][ny][nx] is overwritten. */
+void lco_forward_project_adjoint(const lco_ct_geom *g, const float *angles, const float *couch,
+                                 long view0, long n_views, const double *proj,
+                                 const lco_img_geom *ig, double z0, int nz, double step,
+                                 double *vol);
+
 int lco_num_threads(void);
 void lco_set_num_threads(int n);
 

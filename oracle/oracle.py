@@ -113,6 +113,9 @@ def lib():
                                    f64p]
         L.lco_forward_project.argtypes = [P(CTGeom), f32p, f32p, C.c_long, C.c_long, f64p,
                                           P(ImgGeom), C.c_double, C.c_int, C.c_double, f64p]
+        L.lco_forward_project_adjoint.argtypes = [P(CTGeom), f32p, f32p, C.c_long, C.c_long,
+                                                  f64p, P(ImgGeom), C.c_double, C.c_int,
+                                                  C.c_double, f64p]
         L.lco_num_threads.restype = C.c_int
         L.lco_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -267,6 +270,19 @@ def forward_project(g: CTGeom, angles: np.ndarray, couch: np.ndarray, vol: np.nd
                               _p(v, C.c_double), C.byref(ig), float(z0), v.shape[0],
                               float(step), _p(out, C.c_double))
     return out
+
+
+def forward_project_adjoint(g: CTGeom, angles: np.ndarray, couch: np.ndarray,
+                            proj: np.ndarray, ig: ImgGeom, z0: float, nz: int, step: float,
+                            view0: int = 0) -> np.ndarray:
+    """Transpose of forward_project: proj [n_views][nt][ns] -> vol [nz][ny][nx]."""
+    a, c = _f32(angles), _f32(couch)
+    p = _f64(proj)
+    vol = np.empty((nz, ig.ny, ig.nx), dtype=np.float64)
+    lib().lco_forward_project_adjoint(C.byref(g), _p(a, C.c_float), _p(c, C.c_float), view0,
+                                      p.shape[0], _p(p, C.c_double), C.byref(ig), float(z0), nz,
+                                      float(step), _p(vol, C.c_double))
+    return vol
 
 
 def voxelize(ig: ImgGeom, z0: float, nz: int, ell: np.ndarray, supersample: int = 1):

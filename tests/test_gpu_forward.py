@@ -113,3 +113,53 @@ def test_random_volume_matches_the_oracle(ctx):
     # FP32 sums; a ray whose chord / step sits within rounding of an integer
     # may take one step more or fewer than the FP64 restatement
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-4
+
+
+def gpu_adjoint(ctx, g, ig, z0, proj, ang, zs, step, nz):
+    cg = _lib.CTGeom.from_buffer_copy(bytes(g))
+    vg = _lib.ImgGeom.from_buffer_copy(bytes(ig))
+    out = torch.full((nz, ig.ny, ig.nx), 7.0, dtype=torch.float32, device="cuda")  # overwritten
+    p = torch.from_numpy(np.ascontiguousarray(proj, np.float32)).cuda()
+    ctx.forward_project_adjoint(cg, ang, zs, p, vg, z0, out, view0=0, step=step)
+    ctx.synchronize()
+    return out.cpu().numpy().astype(np.float64)
+
+
+def test_adjoint_matches_the_oracle_and_pairs_with_the_projector(ctx):
+    """The FP32 adjoint against the FP64 transpose (itself pinned by the
+    explicit matrix), the dot-product identity of the GPU pair, and a second,
+    smaller call that must overwrite rather than accumulate."""
+    from test_oracle_phantom import adjoint_case
+    g, ig, z0, ang, zs = adjoint_case(nx=20, ny=18, nz=12)
+    rng = np.random.default_rng(7)
+    y = rng.standard_normal((ang.size, g.nt, g.ns)).astype(np.float32)
+    x = rng.standard_normal((ig.nz, ig.ny, ig.nx)).astype(np.float32)
+    got = gpu_adjoint(ctx, g, ig, z0, y, ang, zs, 1.3, ig.nz)
+    ref = O.forward_project_adjoint(g, ang, zs, y, ig, z0, ig.nz, step=1.3)
+    assert np.abs(ref).max() > 1.0
+    # as for the projector: a ray whose chord / step sits within rounding of
+    # an integer may take one step more or fewer than the FP64 restatement
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-4
+    ax = gpu_forward(ctx, g, ig, z0, x, ang, zs, 1.3)
+    lhs = float((ax * y).sum())
+    rhs = float((x.astype(np.float64) * got).sum())
+    assert abs(lhs) > 1.0
+    assert lhs == pytest.approx(rhs, rel=1e-5)
+    got2 = gpu_adjoint(ctx, g, ig, z0, y[:2], ang[:2], zs[:2], 1.3, ig.nz)
+    ref2 = O.forward_project_adjoint(g, ang[:2], zs[:2], y[:2], ig, z0, ig.nz, step=1.3)
+    assert np.linalg.norm(got2 - ref2) / np.linalg.norm(ref2) < 1e-4
+
+
+def test_projector_wrappers_refuse_mismatched_buffers(ctx):
+    from test_oracle_phantom import adjoint_case
+    g, ig, z0, ang, zs = adjoint_case()
+    cg = _lib.CTGeom.from_buffer_copy(bytes(g))
+    vg = _lib.ImgGeom.from_buffer_copy(bytes(ig))
+    vol = torch.zeros((ig.nz, ig.ny + 1, ig.nx), device="cuda")
+    out = torch.zeros((ang.size, g.nt, g.ns), device="cuda")
+    with pytest.raises(ValueError):
+        ctx.forward_project(cg, ang, zs, vol, vg, z0, out, step=1.0)
+    with pytest.raises(ValueError):
+        ctx.forward_project_adjoint(cg, ang, zs, out, vg, z0, vol, step=1.0)
+    with pytest.raises(ValueError):
+        ctx.forward_project(cg, ang, zs, vol[:, :ig.ny], vg, z0, out.double(), step=1.0)

@@ -122,3 +122,47 @@ def test_forward_project_step_independent_for_linear_volumes():
     a = O.forward_project(g, ang, zs, vol, ig, z0, step=0.3)
     b = O.forward_project(g, ang, zs, vol, ig, z0, step=4.0)
     np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-10)
+
+
+def adjoint_case(nx=6, ny=5, nz=4):
+    g = geom(ns=11, nt=4, ds=9.0, dt=8.0)
+    ig = O.ImgGeom()
+    ig.nx, ig.ny, ig.nz = nx, ny, nz
+    ig.dx = ig.dy = 7.0
+    ig.dz = 6.0
+    ig.wx, ig.wy = (nx - 1) / 2.0 + 0.3, (ny - 1) / 2.0 - 0.4
+    ang = np.array([0.2, 1.3, 2.9, 4.4], np.float32)
+    zs = np.array([-3.0, 4.0, 9.0, -11.0], np.float32)
+    return g, ig, -8.0, ang, zs
+
+
+def test_forward_project_adjoint_equals_the_explicit_transpose():
+    """Brute force: the columns of A are the reprojections of unit volumes;
+    the adjoint of a random sinogram must equal A^T times it."""
+    g, ig, z0, ang, zs = adjoint_case()
+    n_vox = ig.nz * ig.ny * ig.nx
+    cols = []
+    for i in range(n_vox):
+        e = np.zeros(n_vox)
+        e[i] = 1.0
+        cols.append(O.forward_project(g, ang, zs, e.reshape(ig.nz, ig.ny, ig.nx), ig, z0,
+                                      step=2.5).ravel())
+    A = np.stack(cols, axis=1)
+    assert np.count_nonzero(A) > 500  # most rays cross the box
+    y = np.random.default_rng(5).standard_normal(A.shape[0])
+    got = O.forward_project_adjoint(g, ang, zs, y.reshape(ang.size, g.nt, g.ns), ig, z0,
+                                    ig.nz, step=2.5)
+    np.testing.assert_allclose(got.ravel(), A.T @ y, rtol=1e-12, atol=1e-12)
+
+
+def test_forward_project_adjoint_dot_product_identity():
+    """<A x, y> = <x, A^T y> for random x and y on a larger volume."""
+    g, ig, z0, ang, zs = adjoint_case(nx=20, ny=18, nz=12)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((ig.nz, ig.ny, ig.nx))
+    y = rng.standard_normal((ang.size, g.nt, g.ns))
+    ax = O.forward_project(g, ang, zs, x, ig, z0, step=1.3)
+    aty = O.forward_project_adjoint(g, ang, zs, y, ig, z0, ig.nz, step=1.3)
+    lhs, rhs = float((ax * y).sum()), float((x * aty).sum())
+    assert abs(lhs) > 1.0
+    assert lhs == pytest.approx(rhs, rel=1e-12)
