@@ -22,6 +22,21 @@ def _fptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
+def _check_buf(t, n: int, name: str, dtype=None) -> None:
+    """A contiguous CUDA tensor of `dtype` (float32 by default) holding at
+    least n elements (what the kernel will index)."""
+    import torch
+    dtype = torch.float32 if dtype is None else dtype
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype or t.numel() < int(n):
+        got = f"{t.dtype} x {t.numel()}" if isinstance(t, torch.Tensor) else type(t).__name__
+        raise ValueError(f"{name}: expected {dtype} with at least {int(n)} elements, got {got}")
+
+
+def _storage_dtype(storage: int):
+    import torch
+    return {_lib.F32: torch.float32, _lib.F16: torch.float16, _lib.BF16: torch.bfloat16}[storage]
+
+
 def _check_f32(t, shape, name: str) -> None:
     """A float32 tensor of exactly `shape` (what the kernel will index)."""
     import torch
@@ -185,19 +200,29 @@ class Context:
 
     # -------------------------------------------------------- stages
     def rebin(self, kind: int, weight: int, g: _lib.CTGeom, fan, par) -> None:
+        _check_buf(fan, g.n_angle_add * g.nt * g.ns, "fan")
+        _check_buf(par, g.n_angle * g.nt * g.ns, "par")
         check(lib().lct_rebin(self.h, kind, weight, C.byref(g), _dev(fan), _dev(par)))
 
     def weighting(self, kind: int, g: _lib.CTGeom, par) -> None:
+        _check_buf(par, g.n_angle * g.nt * g.ns, "par")
         check(lib().lct_weighting(self.h, kind, C.byref(g), _dev(par)))
 
     def filter(self, g: _lib.CTGeom, taps: np.ndarray, rows, n_rows: int) -> None:
         t = np.ascontiguousarray(taps, np.float32)
+        if t.size < 2 * g.ns:
+            raise ValueError("taps needs 2 ns values")
+        _check_buf(rows, n_rows * g.ns, "rows")
         check(lib().lct_filter(self.h, C.byref(g), _fptr(t), _dev(rows), n_rows))
 
     def filter_to_bp_layout(self, algo: int, storage: int, ns: int, nt: int, taps: np.ndarray,
                             rows, views: int, q) -> None:
         """Ramp filter of rows [views][nt][ns] into Q [views][ns+3][nt+8] (q pre-zeroed)."""
         t = np.ascontiguousarray(taps, np.float32)
+        if t.size < 2 * ns:
+            raise ValueError("taps needs 2 ns values")
+        _check_buf(rows, views * nt * ns, "rows")
+        _check_buf(q, views * (ns + 3) * (nt + 8), "q", _storage_dtype(storage))
         check(lib().lct_filter_to_bp_layout(self.h, algo, storage, ns, nt, _fptr(t), _dev(rows),
                                             views, _dev(q)))
 
@@ -205,6 +230,8 @@ class Context:
         b = np.ascontiguousarray(betas, np.float32)
         if b.size < g.n_angle + g.n_proj_turn // 2 + 1:
             raise ValueError("betas needs n_angle + n_proj_turn/2 + 1 values")
+        _check_buf(q, g.n_angle * g.nt * g.ns, "q")
+        _check_buf(img, ig.nz * ig.ny * ig.nx, "img")
         check(lib().lct_backproject(self.h, kind, C.byref(g), C.byref(ig), _dev(q), _fptr(b),
                                     _dev(img)))
 
@@ -217,10 +244,13 @@ class Context:
         a = np.ascontiguousarray(angles, np.float32)
         c = np.ascontiguousarray(couch, np.float32)
         g = plan.ctg
+        _check_tables(plan, a, c)
+        _check_buf(fan, g.nt * g.ns, "fan")
         views = fan.numel() // (g.nt * g.ns)
         if vol is None:
             vol = torch.empty((plan.n_slices_recon, plan.ig.ny, plan.ig.nx), dtype=torch.float32,
                               device=fan.device)
+        _check_f32(vol, (plan.n_slices_recon, plan.ig.ny, plan.ig.nx), "vol")
         oc = opts.to_c()
         check(lib().lct_reconstruct(self.h, C.byref(plan), C.byref(oc), _fptr(a), _fptr(c),
                                     _dev(fan), views, _dev(vol)))
@@ -233,6 +263,7 @@ class Context:
         g = plan.ctg
         if out is None:
             out = torch.empty((g.n_angle, g.nt, g.ns), dtype=torch.float32, device="cuda")
+        _check_f32(out, (g.n_angle, g.nt, g.ns), "out")
         check(lib().lct_last_block_filtered(self.h, _dev(out)))
         return out
 
@@ -246,10 +277,13 @@ class Context:
         a = np.ascontiguousarray(angles, np.float32)
         c = np.ascontiguousarray(couch, np.float32)
         g = plan.ctg
+        _check_tables(plan, a, c)
+        _check_buf(fan, g.nt * g.ns, "fan")
         views = fan.numel() // (g.nt * g.ns)
         if vol is None:
             vol = torch.empty(((block_end - block_begin) * 32, plan.ig.ny, plan.ig.nx),
                               dtype=torch.float32, device=fan.device)
+        _check_f32(vol, ((block_end - block_begin) * 32, plan.ig.ny, plan.ig.nx), "vol")
         oc = opts.to_c()
         check(lib().lct_reconstruct_slabs(self.h, C.byref(plan), C.byref(oc), _fptr(a), _fptr(c),
                                           _dev(fan), fan_view0, views, block_begin, block_end,
@@ -264,6 +298,9 @@ class Context:
         e = np.ascontiguousarray(ell, np.float64).reshape(-1, 16)
         if n_views is None:
             n_views = a.size - view0
+        if view0 < 0 or n_views < 0 or view0 + n_views > min(a.size, c.size):
+            raise ValueError("views outside the angle / couch tables")
+        _check_buf(out, n_views * g.nt * g.ns, "out")
         check(lib().lct_project_ellipsoids(self.h, C.byref(g), _fptr(a), _fptr(c), view0,
                                            n_views, e.ctypes.data_as(C.POINTER(C.c_double)),
                                            e.shape[0], _dev(out)))
@@ -301,13 +338,22 @@ class Context:
     def voxelize(self, ig: _lib.ImgGeom, z0: float, nz: int, ell: np.ndarray, out,
                  supersample: int = 2) -> None:
         e = np.ascontiguousarray(ell, np.float64).reshape(-1, 16)
+        _check_buf(out, nz * ig.ny * ig.nx, "out")
         check(lib().lct_voxelize_ellipsoids(self.h, C.byref(ig), float(z0), nz,
                                             e.ctypes.data_as(C.POINTER(C.c_double)), e.shape[0],
                                             supersample, _dev(out)))
 
     def poisson_noise(self, proj, i0: float, seed: int) -> None:
+        _check_buf(proj, proj.numel(), "proj")
         check(lib().lct_poisson_noise(self.h, _dev(proj), proj.numel(), C.c_float(i0),
                                       C.c_uint64(seed)))
+
+
+def _check_tables(plan: _lib.Plan, angles: np.ndarray, couch: np.ndarray) -> None:
+    """The planner read plan.n_readings angles and couch positions; the
+    pipeline reads the same."""
+    if angles.size < plan.n_readings or couch.size < plan.n_readings:
+        raise ValueError(f"angle / couch tables shorter than the plan's {plan.n_readings} readings")
 
 
 def scan_geometry(w) -> _lib.CTGeom:
@@ -380,11 +426,15 @@ class Cnn:
         if y is None:
             y = torch.empty_like(x)
         B, D, H, W = x.shape
+        _check_f32(x, (B, D, H, W), "x")
+        _check_f32(y, (B, D, H, W), "y")
         check(lib().lct_cnn_forward(self.h, _dev(x), B, D, H, W, _dev(y)))
         return y
 
     def loss_grad(self, x, t) -> float:
         B, D, H, W = x.shape
+        _check_f32(x, (B, D, H, W), "x")
+        _check_f32(t, (B, D, H, W), "t")
         loss = np.zeros(1, np.float32)
         check(lib().lct_cnn_loss_grad(self.h, _dev(x), _dev(t), B, D, H, W, _fptr(loss)))
         return float(loss[0])
