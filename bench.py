@@ -76,7 +76,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(w, fan_np, a, c, sample_slices=(0, 11, 22)):
+def cpu_baseline(w, fan_np, a, c, sample_slices=tuple(range(0, 32, 2))):
     """FP64 oracle on the host: block 0 (rebin + filter of its whole window, and
     `sample_slices` backprojected), scaled to a whole volume, on all host
     threads; and a single-thread figure from smaller sub-samples."""
