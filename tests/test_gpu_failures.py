@@ -56,3 +56,34 @@ def test_non_finite_views_stay_in_their_blocks(ctx, poisoned, preset):
     ctx.synchronize()
     v = vol.cpu().numpy()
     assert [bool(np.isfinite(v[32 * b:32 * b + 32]).all()) for b in range(3)] == finite
+
+
+def test_unsupported_shapes_are_refused_before_any_launch(ctx):
+    """Shapes a kernel cannot take are refused on the host with
+    LCT_ERR_UNSUPPORTED, and the context keeps working."""
+    # reduced-precision storage needs the pair-folded backprojector (even npt)
+    w = scan.workload("tiny", n_proj_turn=97, readings=1200)
+    a, c = w.angles_couch()
+    pl = recon.plan_helical(w.prm(), a, c)
+    fan = torch.from_numpy(cpu_fan(w)).cuda()
+    print("odd npt with f16 storage", flush=True)
+    with pytest.raises(_lib.LctError) as e:
+        ctx.reconstruct(pl, recon.preset("mixed_f16", window=w.window), a, c, fan)
+    assert e.value.code == _lib.LCT_ERR_UNSUPPORTED and "pair-folded" in str(e.value)
+    # the Toeplitz filter needs ns % 4 == 0
+    ns, nt = 30, 8
+    rows = torch.zeros(nt * ns, device="cuda")
+    q = torch.zeros((ns + 3) * (nt + 8), dtype=torch.float16, device="cuda")
+    with pytest.raises(_lib.LctError) as e:
+        ctx.filter_to_bp_layout(_lib.FILTER_TOEPLITZ_BF16X3, _lib.F16, ns, nt,
+                                np.zeros(2 * ns, np.float32), rows, 1, q)
+    assert e.value.code == _lib.LCT_ERR_UNSUPPORTED
+    # the direct FP32 filter holds one row per workgroup: ns <= 2048
+    g = _lib.CTGeom()
+    g.ns, g.nt, g.n_angle, g.ds = 2050, 1, 1, 1.0
+    with pytest.raises(_lib.LctError) as e:
+        ctx.filter(g, np.zeros(2 * g.ns, np.float32), torch.zeros(g.ns, device="cuda"), 1)
+    assert e.value.code == _lib.LCT_ERR_UNSUPPORTED
+    vol = ctx.reconstruct(pl, recon.preset("fast_f32", window=w.window), a, c, fan)
+    ctx.synchronize()
+    assert torch.isfinite(vol).all()
