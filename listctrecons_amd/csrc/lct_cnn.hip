@@ -118,19 +118,36 @@ __global__ __launch_bounds__(256) void conv_out1_kernel(const bf16_t *__restrict
     const int xx = int(v % s.W), yy = int((v / s.W) % s.H), zz = int((v / (long(s.W) * s.H)) % s.D);
     const int b = int(v / (long(s.W) * s.H * s.D));
     float acc = bias[0];
-    for (int t = 0; t < kTaps; t++) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        if (!inside(s, zz + dz, yy + dy, xx + dx)) continue;
-        const uint4 *src = reinterpret_cast<const uint4 *>(h + vox(s, b, zz + dz, yy + dy, xx + dx) * kC);
+    // one (dz, dy) row of three taps at a time: its twelve 16-byte loads (from
+    // clamped, always valid addresses) are in flight together; taps outside
+    // the volume are zero padding and are skipped when accumulating
+    for (int r = 0; r < 9; r++) {
+        const int dz = r / 3 - 1, dy = r % 3 - 1;
+        const int gz = zz + dz, gy = yy + dy;
+        const bool row_in = gz >= 0 && gz < s.D && gy >= 0 && gy < s.H;
+        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
+        uint4 q[3][4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const uint4 q = src[k];
-            const unsigned wv[4] = {q.x, q.y, q.z, q.w};
+        for (int i = 0; i < 3; i++) {
+            const int cx = min(max(xx + i - 1, 0), s.W - 1);
+            const uint4 *src = reinterpret_cast<const uint4 *>(h + vox(s, b, cz, cy, cx) * kC);
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int ci = k * 8 + e * 2;
-                acc = fmaf(sw[t * kC + ci], bf2f(bf16_t(wv[e] & 0xffffu)), acc);
-                acc = fmaf(sw[t * kC + ci + 1], bf2f(bf16_t(wv[e] >> 16)), acc);
+            for (int k = 0; k < 4; k++) q[i][k] = src[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int gx = xx + i - 1;
+            if (!row_in || gx < 0 || gx >= s.W) continue;
+            const int t = r * 3 + i;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const unsigned wv[4] = {q[i][k].x, q[i][k].y, q[i][k].z, q[i][k].w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int ci = k * 8 + e * 2;
+                    acc = fmaf(sw[t * kC + ci], bf2f(bf16_t(wv[e] & 0xffffu)), acc);
+                    acc = fmaf(sw[t * kC + ci + 1], bf2f(bf16_t(wv[e] >> 16)), acc);
+                }
             }
         }
     }
@@ -166,15 +183,29 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
     const int b = int(t / tiles_z);
     const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
 
-    // stage the halo: 648 voxels x 4 chunks of 16 bytes
-    for (int c = threadIdx.x; c < kHalo * 4; c += 256) {
+    // stage the halo: 648 voxels x 4 chunks of 16 bytes, every load of a
+    // thread issued before its LDS writes (one memory latency, not eleven)
+    // (loads from clamped, always valid addresses; the zero padding is
+    // applied when the chunk is written)
+    constexpr int kStage = (kHalo * 4 + 255) / 256;
+    uint4 val[kStage];
+    bool pad[kStage];
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int c = min(int(threadIdx.x) + 256 * i, kHalo * 4 - 1);
         const int hv = c >> 2, part = c & 3;
         const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
         const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
-        uint4 val = make_uint4(0u, 0u, 0u, 0u);
-        if (inside(s, gz, gy, gx))
-            val = reinterpret_cast<const uint4 *>(in + vox(s, b, gz, gy, gx) * kC)[part];
-        reinterpret_cast<uint4 *>(halo)[c] = val;
+        pad[i] = !inside(s, gz, gy, gx);
+        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
+        const int cx = min(max(gx, 0), s.W - 1);
+        val[i] = reinterpret_cast<const uint4 *>(in + vox(s, b, cz, cy, cx) * kC)[part];
+    }
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int c = int(threadIdx.x) + 256 * i;
+        if (c < kHalo * 4)
+            reinterpret_cast<uint4 *>(halo)[c] = pad[i] ? make_uint4(0u, 0u, 0u, 0u) : val[i];
     }
     __syncthreads();
 
@@ -186,12 +217,18 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 #pragma unroll
         for (int f = 0; f < 2; f++) acc[m][f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
-    for (int tap = 0; tap < kTaps; tap++) {
-        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        bf16x8_t bfrag[2];
+    // weight fragments of tap + 1 are loaded while tap's products run
+    auto load_b = [&](int tap, bf16x8_t (&bf)[2]) {
 #pragma unroll
         for (int f = 0; f < 2; f++)
-            bfrag[f] = *reinterpret_cast<const bf16x8_t *>(w + (size_t(tap) * kC + f * 16 + li) * kC + lk);
+            bf[f] = *reinterpret_cast<const bf16x8_t *>(w + (size_t(tap) * kC + f * 16 + li) * kC + lk);
+    };
+    bf16x8_t bnext[2];
+    load_b(0, bnext);
+    for (int tap = 0; tap < kTaps; tap++) {
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const bf16x8_t bfrag[2] = {bnext[0], bnext[1]};
+        if (tap + 1 < kTaps) load_b(tap + 1, bnext);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
             const int hz = wave + 1 + dz, hy = m + 1 + dy, hx = li + 1 + dx;
