@@ -11,8 +11,10 @@
 // The 32 -> 32 layers run as implicit GEMMs on v_mfma_f32_16x16x32_bf16
 // (M = voxels, N = output channels, K = one tap's 32 input channels per
 // step, 27 steps), bf16 operands, FP32 accumulation.  Their data gradient is
-// the same kernel with the taps mirrored and W transposed per tap.  The 1- and
-// 32 -> 1 layers and all weight gradients run on FP32 VALU.
+// the same kernel with the taps mirrored and W transposed per tap, and the
+// 32 -> 1 output layer is the same kernel with one N fragment whose column 0
+// holds the (bf16) output weights.  The 1 -> 32 layer and all weight
+// gradients run on FP32 VALU.
 #include <hip/hip_bf16.h>
 
 #include <cmath>
@@ -101,65 +103,14 @@ __global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__
     for (int k = 0; k < 4; k++) dst[k] = outv[k];
 }
 
-// ------------------------------------------ layer 6 (32 -> 1) + residual
-// y[v] = x[v] + b + sum_tap sum_ci w[tap][ci] h[v + off(tap)][ci]
-__global__ __launch_bounds__(256) void conv_out1_kernel(const bf16_t *__restrict__ h,
-                                                        const float *__restrict__ w,
-                                                        const float *__restrict__ bias,
-                                                        const float *__restrict__ x,
-                                                        float *__restrict__ y, Shape s)
-{
-    __shared__ float sw[kTaps * kC];
-    for (int i = threadIdx.x; i < kTaps * kC; i += blockDim.x) sw[i] = w[i];
-    __syncthreads();
-    const long n = long(s.B) * s.D * s.H * s.W;
-    const long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const int xx = int(v % s.W), yy = int((v / s.W) % s.H), zz = int((v / (long(s.W) * s.H)) % s.D);
-    const int b = int(v / (long(s.W) * s.H * s.D));
-    float acc = bias[0];
-    // one (dz, dy) row of three taps at a time: its twelve 16-byte loads (from
-    // clamped, always valid addresses) are in flight together; taps outside
-    // the volume are zero padding and are skipped when accumulating
-    for (int r = 0; r < 9; r++) {
-        const int dz = r / 3 - 1, dy = r % 3 - 1;
-        const int gz = zz + dz, gy = yy + dy;
-        const bool row_in = gz >= 0 && gz < s.D && gy >= 0 && gy < s.H;
-        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
-        uint4 q[3][4];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int cx = min(max(xx + i - 1, 0), s.W - 1);
-            const uint4 *src = reinterpret_cast<const uint4 *>(h + vox(s, b, cz, cy, cx) * kC);
-#pragma unroll
-            for (int k = 0; k < 4; k++) q[i][k] = src[k];
-        }
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int gx = xx + i - 1;
-            if (!row_in || gx < 0 || gx >= s.W) continue;
-            const int t = r * 3 + i;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const unsigned wv[4] = {q[i][k].x, q[i][k].y, q[i][k].z, q[i][k].w};
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int ci = k * 8 + e * 2;
-                    acc = fmaf(sw[t * kC + ci], bf2f(bf16_t(wv[e] & 0xffffu)), acc);
-                    acc = fmaf(sw[t * kC + ci + 1], bf2f(bf16_t(wv[e] >> 16)), acc);
-                }
-            }
-        }
-    }
-    y[v] = x[v] + acc;
-}
-
 // -------------------------------------------- 32 -> 32 layers on MFMA
 // Workgroup tile: 4 (z) x 4 (y) x 16 (x) output voxels of one batch item;
 // wave w owns plane z0 + w (4 M-fragments of 16 voxels) and all 32 output
 // channels (2 N-fragments).  The (4+2) x (4+2) x (16+2) halo of input
 // voxels (32 bf16 channels each) is staged in LDS.
-// mode 0: out = relu(conv + bias); mode 1: out = conv * (mask > 0).
+// mode 0: out = relu(conv + bias); mode 1: out = conv * (mask > 0);
+// mode 2: yout = xres + bias[0] + conv, one output channel (w is [27][16][32]
+// with only column 0 non-zero; one N fragment).
 constexpr int kTZ = 4, kTY = 4, kTX = 16;
 constexpr int kHZ = kTZ + 2, kHY = kTY + 2, kHX = kTX + 2;
 constexpr int kHalo = kHZ * kHY * kHX;  // 648 voxels
@@ -169,8 +120,11 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
                                                        const bf16_t *__restrict__ w,  // [27][n][k]
                                                        const float *__restrict__ bias,
                                                        const bf16_t *__restrict__ mask,
-                                                       bf16_t *__restrict__ out, Shape s)
+                                                       bf16_t *__restrict__ out, Shape s,
+                                                       const float *__restrict__ xres,
+                                                       float *__restrict__ yout)
 {
+    constexpr int NF = MODE == 2 ? 1 : 2;  // N fragments of 16 output channels
     __shared__ __attribute__((aligned(16))) bf16_t halo[kHalo * kC];  // 41472 B
     const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
     const int tiles_z = (s.D + kTZ - 1) / kTZ;
@@ -211,23 +165,26 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = (lane >> 4) * 8;
-    f32x4_t acc[4][2];
+    f32x4_t acc[4][NF];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-        for (int f = 0; f < 2; f++) acc[m][f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int f = 0; f < NF; f++) acc[m][f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
     // weight fragments of tap + 1 are loaded while tap's products run
-    auto load_b = [&](int tap, bf16x8_t (&bf)[2]) {
+    constexpr int wn = NF * 16;  // rows (output channels) per tap of w
+    auto load_b = [&](int tap, bf16x8_t (&bf)[NF]) {
 #pragma unroll
-        for (int f = 0; f < 2; f++)
-            bf[f] = *reinterpret_cast<const bf16x8_t *>(w + (size_t(tap) * kC + f * 16 + li) * kC + lk);
+        for (int f = 0; f < NF; f++)
+            bf[f] = *reinterpret_cast<const bf16x8_t *>(w + (size_t(tap) * wn + f * 16 + li) * kC + lk);
     };
-    bf16x8_t bnext[2];
+    bf16x8_t bnext[NF];
     load_b(0, bnext);
     for (int tap = 0; tap < kTaps; tap++) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        const bf16x8_t bfrag[2] = {bnext[0], bnext[1]};
+        bf16x8_t bfrag[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) bfrag[f] = bnext[f];
         if (tap + 1 < kTaps) load_b(tap + 1, bnext);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
@@ -235,7 +192,7 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
             const bf16x8_t afrag =
                 *reinterpret_cast<const bf16x8_t *>(halo + ((hz * kHY + hy) * kHX + hx) * kC + lk);
 #pragma unroll
-            for (int f = 0; f < 2; f++)
+            for (int f = 0; f < NF; f++)
                 acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[f], acc[m][f], 0, 0, 0);
         }
     }
@@ -244,7 +201,7 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-        for (int f = 0; f < 2; f++)
+        for (int f = 0; f < NF; f++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int xl = (lane >> 4) * 4 + j;                 // voxel row within the fragment
@@ -252,6 +209,17 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
                 tile[vl * kC + f * 16 + li] = acc[m][f][j];
             }
     __syncthreads();
+    if constexpr (MODE == 2) {
+        // one output voxel per thread: channel 0 of the tile
+        const int vl = int(threadIdx.x);
+        const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
+        const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
+        if (inside(s, gz, gy, gx)) {
+            const long gv = vox(s, b, gz, gy, gx);
+            yout[gv] = xres[gv] + (bias[0] + tile[vl * kC]);
+        }
+        return;
+    }
     for (int c = threadIdx.x; c < kTZ * kTY * kTX * 4; c += 256) {
         const int vl = c >> 2, part = c & 3;
         const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
@@ -481,7 +449,8 @@ __global__ __launch_bounds__(256) void adam_kernel(float *__restrict__ w, float 
 __global__ __launch_bounds__(256) void weights_to_operands_kernel(const float *__restrict__ w,
                                                                   bf16_t *__restrict__ wb,
                                                                   bf16_t *__restrict__ wt,
-                                                                  float *__restrict__ wio)
+                                                                  float *__restrict__ wio,
+                                                                  bf16_t *__restrict__ w6op)
 {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long n_mid = 4L * kTaps * kC * kC;
@@ -499,6 +468,11 @@ __global__ __launch_bounds__(256) void weights_to_operands_kernel(const float *_
         wio[i] = bf2f(f2bf(w[kW1 + i]));
         wio[kTaps * kC + i] = bf2f(f2bf(w[kW6 + i]));
     }
+    // output layer as an MFMA operand [27][16][32]: column 0 = W6[tap][ci]
+    if (i < long(kTaps) * 16 * kC) {
+        const int tap = int(i / (16 * kC)), n = int((i / kC) % 16), ci = int(i % kC);
+        w6op[i] = n == 0 ? f2bf(w[kW6 + tap * kC + ci]) : bf16_t(0);
+    }
 }
 
 }  // namespace lct
@@ -509,6 +483,7 @@ struct lct_cnn {
     lct_ctx *ctx = nullptr;
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 [kParams]
     bf16_t *wb = nullptr, *wt = nullptr;                             // 4 x 27 x 32 x 32
+    bf16_t *w6op = nullptr;                                          // 27 x 16 x 32
     float *wio = nullptr;                                            // rounded W1, W6
     bf16_t *act[6] = {};   // h1..h5 (index 1..5)
     bf16_t *grad[2] = {};  // ping-pong gradient activations
@@ -541,7 +516,7 @@ int refresh_operands(lct_cnn *n)
 {
     const long cnt = 4L * kTaps * kC * kC;
     hipLaunchKernelGGL(weights_to_operands_kernel, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0,
-                       n->ctx->stream, n->w, n->wb, n->wt, n->wio);
+                       n->ctx->stream, n->w, n->wb, n->wt, n->wio, n->w6op);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -559,9 +534,10 @@ int forward(lct_cnn *n, const float *x, Shape s, float *y)
     for (int l = 0; l < 4; l++)
         hipLaunchKernelGGL(conv_mid_kernel<0>, dim3(unsigned(tiles)), dim3(256), 0, st,
                            n->act[l + 1], n->wb + size_t(l) * kTaps * kC * kC,
-                           n->w + kMid + l * kMidStride + kTaps * kC * kC, nullptr, n->act[l + 2], s);
-    hipLaunchKernelGGL(conv_out1_kernel, dim3(blocks_for(nv)), dim3(256), 0, st, n->act[5],
-                       n->wio + kTaps * kC, n->w + kB6, x, y, s);
+                           n->w + kMid + l * kMidStride + kTaps * kC * kC, nullptr, n->act[l + 2], s,
+                           nullptr, nullptr);
+    hipLaunchKernelGGL(conv_mid_kernel<2>, dim3(unsigned(tiles)), dim3(256), 0, st, n->act[5],
+                       n->w6op, n->w + kB6, nullptr, nullptr, s, x, y);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -590,7 +566,7 @@ int backward(lct_cnn *n, const float *x, Shape s)
                            n->g + kMid + l * kMidStride + kTaps * kC * kC, nv);
         hipLaunchKernelGGL(conv_mid_kernel<1>, dim3(unsigned(tiles)), dim3(256), 0, st, gk,
                            n->wt + size_t(l) * kTaps * kC * kC, nullptr, n->act[l + 1],
-                           n->grad[cur ^ 1], s);
+                           n->grad[cur ^ 1], s, nullptr, nullptr);
         cur ^= 1;
     }
     // layer 1: grad[cur] = dL/d(pre-activation of layer 1)
@@ -632,6 +608,8 @@ int lct_cnn_create(lct_ctx *ctx, uint64_t seed, lct_cnn **out)
     if ((e = hipMalloc(&n->wb, 4L * kTaps * kC * kC * 2)) != hipSuccess) return fail(e, "hipMalloc(wb)");
     if ((e = hipMalloc(&n->wt, 4L * kTaps * kC * kC * 2)) != hipSuccess) return fail(e, "hipMalloc(wt)");
     if ((e = hipMalloc(&n->wio, 2L * kTaps * kC * 4)) != hipSuccess) return fail(e, "hipMalloc(wio)");
+    if ((e = hipMalloc(&n->w6op, long(kTaps) * 16 * kC * 2)) != hipSuccess)
+        return fail(e, "hipMalloc(w6op)");
     if ((e = hipMalloc(&n->loss, 4)) != hipSuccess) return fail(e, "hipMalloc(loss)");
     // He-normal initialisation from the seed; the last layer starts at 1/100 so
     // the initial correction is close to zero; zero biases.
@@ -663,7 +641,8 @@ int lct_cnn_destroy(lct_cnn *n)
     (void)hipDeviceSynchronize();
     for (void *p : {static_cast<void *>(n->w), static_cast<void *>(n->m), static_cast<void *>(n->v),
                     static_cast<void *>(n->g), static_cast<void *>(n->wb), static_cast<void *>(n->wt),
-                    static_cast<void *>(n->wio), static_cast<void *>(n->loss),
+                    static_cast<void *>(n->wio), static_cast<void *>(n->w6op),
+                    static_cast<void *>(n->loss),
                     static_cast<void *>(n->dy), static_cast<void *>(n->y)})
         if (p) (void)hipFree(p);
     for (auto p : n->act) if (p) (void)hipFree(p);
