@@ -13,8 +13,8 @@
 // step, 27 steps), bf16 operands, FP32 accumulation.  Their data gradient is
 // the same kernel with the taps mirrored and W transposed per tap, and the
 // 32 -> 1 output layer is the same kernel with one N fragment whose column 0
-// holds the (bf16) output weights.  The 1 -> 32 layer and all weight
-// gradients run on FP32 VALU.
+// holds the (bf16) output weights.  The 1 -> 32 layer is an MFMA GEMM over
+// the 27 taps; all weight gradients run on FP32 VALU.
 #include <hip/hip_bf16.h>
 
 #include <cmath>
@@ -64,45 +64,6 @@ __device__ __forceinline__ bool inside(const Shape &s, int z, int y, int x)
     return z >= 0 && z < s.D && y >= 0 && y < s.H && x >= 0 && x < s.W;
 }
 
-// ------------------------------------------------- layer 1 (1 -> 32), VALU
-// h[v][co] = relu(b[co] + sum_tap w[tap][co] x[v + off(tap)])
-__global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__ x,
-                                                       const float *__restrict__ w,
-                                                       const float *__restrict__ bias,
-                                                       bf16_t *__restrict__ h, Shape s)
-{
-    __shared__ float sw[kTaps * kC + kC];
-    for (int i = threadIdx.x; i < kTaps * kC; i += blockDim.x) sw[i] = w[i];
-    for (int i = threadIdx.x; i < kC; i += blockDim.x) sw[kTaps * kC + i] = bias[i];
-    __syncthreads();
-    const long n = long(s.B) * s.D * s.H * s.W;
-    const long v = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v >= n) return;
-    const int xx = int(v % s.W), yy = int((v / s.W) % s.H), zz = int((v / (long(s.W) * s.H)) % s.D);
-    const int b = int(v / (long(s.W) * s.H * s.D));
-    float xt[kTaps];
-#pragma unroll
-    for (int t = 0; t < kTaps; t++) {
-        const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-        xt[t] = inside(s, zz + dz, yy + dy, xx + dx) ? x[vox(s, b, zz + dz, yy + dy, xx + dx)] : 0.0f;
-    }
-    uint4 outv[4];
-    unsigned *ow = reinterpret_cast<unsigned *>(outv);
-#pragma unroll
-    for (int co = 0; co < kC; co += 2) {
-        float a0 = sw[kTaps * kC + co], a1 = sw[kTaps * kC + co + 1];
-#pragma unroll
-        for (int t = 0; t < kTaps; t++) {
-            a0 = fmaf(sw[t * kC + co], xt[t], a0);
-            a1 = fmaf(sw[t * kC + co + 1], xt[t], a1);
-        }
-        ow[co / 2] = unsigned(f2bf(fmaxf(a0, 0.0f))) | (unsigned(f2bf(fmaxf(a1, 0.0f))) << 16);
-    }
-    uint4 *dst = reinterpret_cast<uint4 *>(h + v * kC);
-#pragma unroll
-    for (int k = 0; k < 4; k++) dst[k] = outv[k];
-}
-
 // -------------------------------------------- 32 -> 32 layers on MFMA
 // Workgroup tile: 4 (z) x 4 (y) x 16 (x) output voxels of one batch item;
 // wave w owns plane z0 + w (4 M-fragments of 16 voxels) and all 32 output
@@ -111,9 +72,113 @@ __global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__
 // mode 0: out = relu(conv + bias); mode 1: out = conv * (mask > 0);
 // mode 2: yout = xres + bias[0] + conv, one output channel (w is [27][16][32]
 // with only column 0 non-zero; one N fragment).
-constexpr int kTZ = 4, kTY = 4, kTX = 16;
+constexpr int kTZ = 4, kTY = 4, kTX = 16;  // 2 x 16 and 8 x 16 y-x tiles ran slower
 constexpr int kHZ = kTZ + 2, kHY = kTY + 2, kHX = kTX + 2;
-constexpr int kHalo = kHZ * kHY * kHX;  // 648 voxels
+constexpr int kHalo = kHZ * kHY * kHX;  // 648 voxels, 41472 B of LDS
+
+// ------------------------------------------- layer 1 (1 -> 32) on MFMA
+// h[v][co] = relu(b[co] + sum_tap w[tap][co] x[v + off(tap)]) as a GEMM with
+// K = the 27 taps (padded to 32): the f32 input is split into bf16 hi + lo
+// (two products, ~2^-17 relative), the weights are bf16, FP32 accumulation.
+// Same 4 x 4 x 16 tile as above; the (4+2) x (4+2) x (16+2) f32 halo and the
+// f32 output tile share one LDS buffer.  w: [32 co][32 taps] bf16, taps >= 27 zero.
+__global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__ x,
+                                                       const bf16_t *__restrict__ w,
+                                                       const float *__restrict__ bias,
+                                                       bf16_t *__restrict__ h, Shape s)
+{
+    __shared__ __attribute__((aligned(16))) float lds[kTZ * kTY * kTX * kC];  // 32 KB
+    float *hx_ = lds;  // halo, 648 floats, until the products are done
+    const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
+    const int tiles_z = (s.D + kTZ - 1) / kTZ;
+    long t = blockIdx.x;
+    const int tx = int(t % tiles_x);
+    t /= tiles_x;
+    const int ty = int(t % tiles_y);
+    t /= tiles_y;
+    const int tz = int(t % tiles_z);
+    const int b = int(t / tiles_z);
+    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+
+    constexpr int kStage = (kHalo + 255) / 256;
+    float val[kStage];
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int c = min(int(threadIdx.x) + 256 * i, kHalo - 1);
+        const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
+        const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+        const bool pad = !inside(s, gz, gy, gx);
+        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
+        const int cx = min(max(gx, 0), s.W - 1);
+        const float v = x[vox(s, b, cz, cy, cx)];
+        val[i] = pad ? 0.0f : v;
+    }
+#pragma unroll
+    for (int i = 0; i < kStage; i++) {
+        const int c = int(threadIdx.x) + 256 * i;
+        if (c < kHalo) hx_[c] = val[i];
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    bf16x8_t bfrag[2];
+#pragma unroll
+    for (int f = 0; f < 2; f++)
+        bfrag[f] = *reinterpret_cast<const bf16x8_t *>(w + (f * 16 + li) * kC + lk);
+    f32x4_t acc[kTY][2];
+#pragma unroll
+    for (int m = 0; m < kTY; m++) {
+        // A fragment: voxel (wave, m, li), taps lk .. lk + 7
+        bf16x8_t ahi, alo;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int tap = lk + j;
+            float v = 0.0f;
+            if (tap < kTaps) {
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                v = hx_[((wave + 1 + dz) * kHY + m + 1 + dy) * kHX + li + 1 + dx];
+            }
+            const bf16_t hi = f2bf(v);
+            ahi[j] = __builtin_bit_cast(__bf16, hi);
+            alo[j] = __builtin_bit_cast(__bf16, f2bf(v - bf2f(hi)));
+        }
+#pragma unroll
+        for (int f = 0; f < 2; f++) {
+            acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bfrag[f],
+                                                                 f32x4_t{0.0f, 0.0f, 0.0f, 0.0f}, 0, 0, 0);
+            acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bfrag[f], acc[m][f], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // halo reads done: the buffer becomes the output tile
+    float *tile = lds;  // [256 voxels][32] f32
+#pragma unroll
+    for (int m = 0; m < kTY; m++)
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int vl = (wave * kTY + m) * kTX + (lane >> 4) * 4 + j;
+                tile[vl * kC + f * 16 + li] = acc[m][f][j];
+            }
+    __syncthreads();
+    for (int c = threadIdx.x; c < kTZ * kTY * kTX * 4; c += 256) {
+        const int vl = c >> 2, part = c & 3;
+        const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
+        const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
+        if (!inside(s, gz, gy, gx)) continue;
+        const float *src = tile + vl * kC + part * 8;
+        unsigned o[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int ch = part * 8 + 2 * e;
+            const float a0 = fmaxf(src[2 * e] + bias[ch], 0.0f);
+            const float a1 = fmaxf(src[2 * e + 1] + bias[ch + 1], 0.0f);
+            o[e] = unsigned(f2bf(a0)) | (unsigned(f2bf(a1)) << 16);
+        }
+        reinterpret_cast<uint4 *>(h + vox(s, b, gz, gy, gx) * kC)[part] = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict__ in,
@@ -125,7 +190,7 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
                                                        float *__restrict__ yout)
 {
     constexpr int NF = MODE == 2 ? 1 : 2;  // N fragments of 16 output channels
-    __shared__ __attribute__((aligned(16))) bf16_t halo[kHalo * kC];  // 41472 B
+    __shared__ __attribute__((aligned(16))) bf16_t halo[kHalo * kC];
     const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
     const int tiles_z = (s.D + kTZ - 1) / kTZ;
     long t = blockIdx.x;
@@ -165,9 +230,9 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = (lane >> 4) * 8;
-    f32x4_t acc[4][NF];
+    f32x4_t acc[kTY][NF];
 #pragma unroll
-    for (int m = 0; m < 4; m++)
+    for (int m = 0; m < kTY; m++)
 #pragma unroll
         for (int f = 0; f < NF; f++) acc[m][f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
 
@@ -187,7 +252,7 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
         for (int f = 0; f < NF; f++) bfrag[f] = bnext[f];
         if (tap + 1 < kTaps) load_b(tap + 1, bnext);
 #pragma unroll
-        for (int m = 0; m < 4; m++) {
+        for (int m = 0; m < kTY; m++) {
             const int hz = wave + 1 + dz, hy = m + 1 + dy, hx = li + 1 + dx;
             const bf16x8_t afrag =
                 *reinterpret_cast<const bf16x8_t *>(halo + ((hz * kHY + hy) * kHX + hx) * kC + lk);
@@ -199,7 +264,7 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
     __syncthreads();  // halo no longer needed: reuse it for the f32 output tile
     float *tile = reinterpret_cast<float *>(halo);  // [256 voxels][32] f32 = 32 KB
 #pragma unroll
-    for (int m = 0; m < 4; m++)
+    for (int m = 0; m < kTY; m++)
 #pragma unroll
         for (int f = 0; f < NF; f++)
 #pragma unroll
@@ -210,13 +275,14 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
             }
     __syncthreads();
     if constexpr (MODE == 2) {
-        // one output voxel per thread: channel 0 of the tile
-        const int vl = int(threadIdx.x);
-        const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
-        const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
-        if (inside(s, gz, gy, gx)) {
-            const long gv = vox(s, b, gz, gy, gx);
-            yout[gv] = xres[gv] + (bias[0] + tile[vl * kC]);
+        // channel 0 of the tile, one output voxel per thread and pass
+        for (int vl = int(threadIdx.x); vl < kTZ * kTY * kTX; vl += 256) {
+            const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
+            const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
+            if (inside(s, gz, gy, gx)) {
+                const long gv = vox(s, b, gz, gy, gx);
+                yout[gv] = xres[gv] + (bias[0] + tile[vl * kC]);
+            }
         }
         return;
     }
@@ -450,7 +516,8 @@ __global__ __launch_bounds__(256) void weights_to_operands_kernel(const float *_
                                                                   bf16_t *__restrict__ wb,
                                                                   bf16_t *__restrict__ wt,
                                                                   float *__restrict__ wio,
-                                                                  bf16_t *__restrict__ w6op)
+                                                                  bf16_t *__restrict__ w6op,
+                                                                  bf16_t *__restrict__ w1op)
 {
     const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
     const long n_mid = 4L * kTaps * kC * kC;
@@ -468,6 +535,11 @@ __global__ __launch_bounds__(256) void weights_to_operands_kernel(const float *_
         wio[i] = bf2f(f2bf(w[kW1 + i]));
         wio[kTaps * kC + i] = bf2f(f2bf(w[kW6 + i]));
     }
+    // input layer as an MFMA operand [32 co][32 taps], taps >= 27 zero
+    if (i < long(kC) * kC) {
+        const int co = int(i / kC), tap = int(i % kC);
+        w1op[i] = tap < kTaps ? f2bf(w[kW1 + tap * kC + co]) : bf16_t(0);
+    }
     // output layer as an MFMA operand [27][16][32]: column 0 = W6[tap][ci]
     if (i < long(kTaps) * 16 * kC) {
         const int tap = int(i / (16 * kC)), n = int((i / kC) % 16), ci = int(i % kC);
@@ -484,6 +556,7 @@ struct lct_cnn {
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 [kParams]
     bf16_t *wb = nullptr, *wt = nullptr;                             // 4 x 27 x 32 x 32
     bf16_t *w6op = nullptr;                                          // 27 x 16 x 32
+    bf16_t *w1op = nullptr;                                          // 32 x 32
     float *wio = nullptr;                                            // rounded W1, W6
     bf16_t *act[6] = {};   // h1..h5 (index 1..5)
     bf16_t *grad[2] = {};  // ping-pong gradient activations
@@ -516,7 +589,7 @@ int refresh_operands(lct_cnn *n)
 {
     const long cnt = 4L * kTaps * kC * kC;
     hipLaunchKernelGGL(weights_to_operands_kernel, dim3(unsigned((cnt + 255) / 256)), dim3(256), 0,
-                       n->ctx->stream, n->w, n->wb, n->wt, n->wio, n->w6op);
+                       n->ctx->stream, n->w, n->wb, n->wt, n->wio, n->w6op, n->w1op);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -527,10 +600,10 @@ int forward(lct_cnn *n, const float *x, Shape s, float *y)
 {
     const long nv = long(s.B) * s.D * s.H * s.W;
     hipStream_t st = n->ctx->stream;
-    hipLaunchKernelGGL(conv_in1_kernel, dim3(blocks_for(nv)), dim3(256), 0, st, x, n->wio,
-                       n->w + kB1, n->act[1], s);
     const long tiles = long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) *
                        ((s.W + kTX - 1) / kTX);
+    hipLaunchKernelGGL(conv_in1_kernel, dim3(unsigned(tiles)), dim3(256), 0, st, x, n->w1op,
+                       n->w + kB1, n->act[1], s);
     for (int l = 0; l < 4; l++)
         hipLaunchKernelGGL(conv_mid_kernel<0>, dim3(unsigned(tiles)), dim3(256), 0, st,
                            n->act[l + 1], n->wb + size_t(l) * kTaps * kC * kC,
@@ -610,6 +683,7 @@ int lct_cnn_create(lct_ctx *ctx, uint64_t seed, lct_cnn **out)
     if ((e = hipMalloc(&n->wio, 2L * kTaps * kC * 4)) != hipSuccess) return fail(e, "hipMalloc(wio)");
     if ((e = hipMalloc(&n->w6op, long(kTaps) * 16 * kC * 2)) != hipSuccess)
         return fail(e, "hipMalloc(w6op)");
+    if ((e = hipMalloc(&n->w1op, long(kC) * kC * 2)) != hipSuccess) return fail(e, "hipMalloc(w1op)");
     if ((e = hipMalloc(&n->loss, 4)) != hipSuccess) return fail(e, "hipMalloc(loss)");
     // He-normal initialisation from the seed; the last layer starts at 1/100 so
     // the initial correction is close to zero; zero biases.
@@ -642,6 +716,7 @@ int lct_cnn_destroy(lct_cnn *n)
     for (void *p : {static_cast<void *>(n->w), static_cast<void *>(n->m), static_cast<void *>(n->v),
                     static_cast<void *>(n->g), static_cast<void *>(n->wb), static_cast<void *>(n->wt),
                     static_cast<void *>(n->wio), static_cast<void *>(n->w6op),
+                    static_cast<void *>(n->w1op),
                     static_cast<void *>(n->loss),
                     static_cast<void *>(n->dy), static_cast<void *>(n->y)})
         if (p) (void)hipFree(p);
