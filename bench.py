@@ -228,8 +228,9 @@ def main():
     ap.add_argument("--variants", default="mixed_bf16",
                     help="comma-separated presets to time besides the headline path "
                          "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
-    ap.add_argument("--stream-slabs", type=int, default=0,
-                    help="also time slab streaming from pinned host memory, N blocks per slab")
+    ap.add_argument("--stream-slabs", type=int, default=2,
+                    help="also time slab streaming from pinned host memory, N blocks per slab "
+                         "(0: off)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one device copy of this many GiB first (PMC scale reference)")
     args = ap.parse_args()
