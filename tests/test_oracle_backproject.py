@@ -2,9 +2,9 @@
 
 Pins: conjugate weights sum to one, so a constant sinogram backprojects to
 2 pi x the constant (pi for the PI-window variant, whose window holds npt/2
-views); a literal per-voxel transcription in pure Python agrees with the
-oracle's reorganised loops on a small grid; and a water cylinder reconstructs
-to 2 mu_w (the reference scale; see test_oracle_pipeline).
+views); literal per-voxel transcriptions in pure Python of both kernels agree
+with the oracle's reorganised loops on a small grid; and a water cylinder
+reconstructs to 2 mu_w (the reference scale; see test_oracle_pipeline).
 """
 import math
 
@@ -108,6 +108,69 @@ def test_bp_matches_literal_transcription(pitch):
         for iy in (10, 11):
             for ix in (0, 9, 16, 31):
                 ref = literal_voxel(g, ig, q, betas, ix, iy, iz)
+                assert img[iz, iy, ix] == pytest.approx(ref, rel=1e-10, abs=1e-10)
+
+
+def _roundf(v):
+    """C roundf: halves away from zero."""
+    return int(math.copysign(math.floor(abs(v) + 0.5), v))
+
+
+def literal_voxel_pi(g, ig, q, betas, ix, iy, iz):
+    """Backprojector.cu:171-250 for one voxel, statement by statement: walk
+    back from a_middle while |t| <= P/4, then forward to a_min + npt/2.  The
+    forward walk stops at the block's last view, past which the texture's
+    border makes every term zero."""
+    xc = (ix - ig.wx) * ig.dx
+    yc = (iy - ig.wy) * ig.dy
+    zc = (iz - ig.wz) * ig.dz
+    dzv32 = np.float32(g.pitch) / np.float32(g.n_proj_turn)
+    zc_f = np.float32(np.float32(iz - ig.wz) * np.float32(ig.dz))
+    am = _roundf(float(np.float32((zc_f - np.float32(g.source_z0)) / dzv32)))
+    am = min(max(am, 0), g.n_angle - 1)
+    h = g.n_proj_turn // 2
+    dzv = g.pitch / g.n_proj_turn
+    temp2 = g.pitch / (2 * math.pi)
+
+    def ray(ia):
+        beta = float(betas[ia])
+        s = -xc * math.cos(beta) - yc * math.sin(beta)
+        gamma = math.asin(s / g.dso)
+        temp1 = math.sqrt(g.dso * g.dso - s * s)
+        zoff = zc - (g.source_z0 + dzv * ia)
+        t = temp1 * (zoff - gamma * temp2) / (temp1 + xc * math.sin(beta) - yc * math.cos(beta))
+        t = (t + gamma * temp2) * g.z_dir
+        return s / g.ds + g.ws, t * 2 * (g.nt - 1) / g.pitch + g.wt, t
+
+    total = 0.0
+    ia = am
+    while ia >= 0:
+        p, r, t = ray(ia)
+        if t > g.pitch / 4 or t < -g.pitch / 4:
+            break
+        total += _tap(q, ia, p, r)
+        ia -= 1
+    a_min = ia
+    for ia in range(am + 1, a_min + h + 1):
+        if ia >= g.n_angle:
+            break
+        p, r, _ = ray(ia)
+        total += _tap(q, ia, p, r)
+    return total * math.pi / h
+
+
+@pytest.mark.parametrize("pitch", [1.0, 1.7])
+def test_pi_bp_matches_literal_transcription(pitch):
+    w, pl, betas = small_case(pitch=pitch, readings=1400)
+    g, ig = pl.ctg, pl.ig
+    rng = np.random.default_rng(5)
+    q = rng.normal(size=(g.n_angle, g.nt, g.ns))
+    iz_list = [0, 13, 31]
+    img = O.backproject(g, ig, q, betas, kind=1, iz_list=iz_list, iy_range=(10, 12))
+    for iz in iz_list:
+        for iy in (10, 11):
+            for ix in (0, 9, 16, 31):
+                ref = literal_voxel_pi(g, ig, q, betas, ix, iy, iz)
                 assert img[iz, iy, ix] == pytest.approx(ref, rel=1e-10, abs=1e-10)
 
 
