@@ -103,10 +103,30 @@ def build_oracle() -> str:
     return os.path.join(oracle_dir, "liblct_oracle.so")
 
 
+NATIVE_TESTS = os.path.join(ROOT, "tests", "native")
+
+
+def build_native_tests() -> None:
+    """C++ test programs of the drop-in headers (tests/native/*.cpp) into
+    tests/native/bin/, linked against the built libraries."""
+    if not os.path.isdir(NATIVE_TESTS):
+        return
+    out_dir = os.path.join(NATIVE_TESTS, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(os.listdir(NATIVE_TESTS)):
+        if not f.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, f[:-4])
+        _run(["g++", *CXX_FLAGS, "-o", exe + ".tmp", os.path.join(NATIVE_TESTS, f), f"-L{PKG}",
+              "-lListCTRecons", "-llct", "-Wl,-rpath,$ORIGIN/../../../listctrecons_amd"])
+        os.replace(exe + ".tmp", exe)
+
+
 def build_all(jobs: int = 8) -> None:
     build_oracle()
     build_lct(jobs)
     build_cpp(jobs)
+    build_native_tests()
 
 
 if __name__ == "__main__":

@@ -133,3 +133,13 @@ def test_cli_slab_streaming_and_resume(three_blocks):
     assert "Resuming after block 0" in r.stdout
     assert not os.path.exists(ck)
     assert np.array_equal(image("resumed"), whole)
+
+
+def test_gpu_memory_ctrl_semantics():
+    """GPUMemoryCtrl<T> of the drop-in headers on the device (program built
+    from tests/native/gpu_memory_ctrl.cpp by the package build)."""
+    exe = os.path.join(ROOT, "tests", "native", "bin", "gpu_memory_ctrl")
+    assert os.path.exists(exe), "build the package first"
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout[-2000:]
