@@ -1,8 +1,10 @@
-"""The C++ drop-in's host code (.prm parser, planner, CLI, file and checkpoint
-checks) built with AddressSanitizer and UndefinedBehaviorSanitizer, and run
+"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer.
+
+The C++ drop-in (.prm parser, planner, CLI, file and checkpoint checks) runs
 over the CLI paths that need no GPU: the 12 golden .prm files, --plan,
 missing files, a truncated projection file and foreign or damaged
-checkpoints.  Any sanitizer report fails the test."""
+checkpoints.  The FP64 oracle runs every one of its functions on a tiny scan.
+Any sanitizer report fails the test."""
 import json
 import os
 import shutil
@@ -78,3 +80,23 @@ def test_truncated_file_and_checkpoints(cli, tmp_path):
     for text in ("LISTCTRecons checkpoint 1\nplan 9 9\ndone 1\n", "garbage\n", ""):
         ck.write_text(text)
         assert run(cli, prm, "Helicalfdkramprebingpu-resident-ckpt").returncode == 1
+
+
+def test_oracle_under_sanitizers(tmp_path):
+    """Every oracle function on a tiny scan (tests/native/oracle_sanitize.c),
+    the oracle compiled from its source with ASan and UBSan (and OpenMP)."""
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    exe = str(tmp_path / "oracle_san")
+    cmd = ["gcc", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+           "-fopenmp", "-std=c11", f"-I{os.path.join(ROOT, 'oracle')}",
+           os.path.join(ROOT, "tests", "native", "oracle_sanitize.c"),
+           os.path.join(ROOT, "oracle", "lct_oracle.c"), "-o", exe, "-lm"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-3000:]
+    env = dict(os.environ, OMP_NUM_THREADS="4", ASAN_OPTIONS="detect_leaks=1:halt_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       env=env, timeout=600)
+    assert "Sanitizer" not in r.stdout and "runtime error" not in r.stdout, r.stdout[-3000:]
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout[-3000:]
