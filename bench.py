@@ -26,6 +26,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "s/volume + backproj GUPS, 512²×256: FP32 vs mixed/learned at ≤1 HU RMSE"
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+ENGINE_HZ = 2.4e9  # MI355X peak engine clock
+N_CU, N_SIMD = 256, 1024
 
 
 def log(*a):
@@ -197,15 +199,30 @@ def stream_slabs(ctx, pl, opts, a, c, host_fan, vol, blocks_per_slab, reps=2):
     return best, len(slabs), max_views
 
 
-def load_traffic(kernel_tag):
+def load_profile(kernel_tag):
+    """Per-launch counter figures of a kernel on a workload, committed under
+    profiles/ (rocprofv3 runs); {} when there are none."""
     path = os.path.join(ROOT, "profiles", f"{kernel_tag}.json")
     if not os.path.exists(path):
-        return None
+        return {}
     try:
         with open(path) as f:
-            return json.load(f).get("traffic_bytes_per_launch")
+            return json.load(f)
     except Exception:
+        return {}
+
+
+def issue_busy(prof, launch_s):
+    """VALU and LDS busy fractions of one launch from its SQ counters: wave64
+    FP32 VALU instructions take 2 cycles on a SIMD (1024 SIMDs), LDS-active
+    cycles are per CU (256 CUs), at the 2.4 GHz engine clock."""
+    if "sq_insts_valu_per_launch" not in prof:
         return None
+    cycles = launch_s * ENGINE_HZ
+    return {"valu_busy": round(prof["sq_insts_valu_per_launch"] * 2 / N_SIMD / cycles, 3),
+            "lds_busy": round(prof["sq_lds_idx_active_per_launch"] / N_CU / cycles, 3),
+            "lds_bank_conflict": round(prof["sq_lds_bank_conflict_per_launch"] / N_CU / cycles, 3),
+            "source": "profiles/" + prof.get("sq_source", "")}
 
 
 def hu_stats(x, ref, np, scan):
@@ -367,7 +384,8 @@ def main():
     achieved = alg_bytes / bp_s
     gups = updates / (head["stage_ms"]["backproject_ms"] * 1e-3) / 1e9
     tag = f"bp_fold_{storage}_{w.name}"
-    traffic = load_traffic(tag)
+    prof = load_profile(tag)
+    traffic = prof.get("traffic_bytes_per_launch")
 
     out = {
         "metric": METRIC,
@@ -402,6 +420,8 @@ def main():
             "definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update "
                           f"+ 4 B per voxel written, per launch; {updates / launches:.4g} updates "
                           f"per launch; the taps are read from LDS, so frac > 1 is possible",
+            # what binds the kernel instead: VALU and LDS issue
+            "issue": issue_busy(prof, bp_s),
         },
         "pcie": {"h2d_ms": round(h2d_s * 1e3, 3),
                  "h2d_GBps": round(fan.numel() * 4 / h2d_s / 1e9, 1),
