@@ -169,3 +169,30 @@ def test_poisson_noise_statistics(ctx):
     ctx.poisson_noise(c2, 1e4, 2)
     ctx.synchronize()
     assert torch.equal(a, b) and not torch.equal(a, c2)
+
+
+@pytest.mark.parametrize("ns", [4, 257, 2048])
+def test_filters_at_extreme_widths(ctx, ns):
+    """The narrowest detector, a width that is not a multiple of the 256-lane
+    row, and the widest the direct filter takes (2048 channels, 8 per lane):
+    direct in place, direct into Q and (ns % 4 == 0) Toeplitz into Q."""
+    rng = np.random.default_rng(ns)
+    nt, views = 8, 2
+    g = _lib.CTGeom()
+    g.ns, g.nt, g.n_angle, g.ds = ns, nt, views, 0.5
+    rows = rng.normal(size=(views * nt, ns)).astype(np.float32)
+    taps = recon.ramp_taps(0, ns, 0.5)
+    ref = O.filter_rows(taps.astype(np.float64), rows.astype(np.float64))
+    print(f"filters ns={ns}", flush=True)
+    d = torch.from_numpy(rows.copy()).cuda()
+    ctx.filter(g, taps, d, views * nt)
+    ctx.synchronize()
+    assert rel_rmse(d.cpu().numpy(), ref) < 2e-6
+    for algo, tol in ((0, 2e-6), (1, 1e-4)):
+        if algo == 1 and ns % 4:
+            continue
+        q = torch.zeros((views, ns + 3, nt + 8), dtype=torch.float32, device="cuda")
+        ctx.filter_to_bp_layout(algo, 0, ns, nt, taps, torch.from_numpy(rows).cuda(), views, q)
+        ctx.synchronize()
+        got = q.cpu().numpy()[:, 1:ns + 1, 4:nt + 4].transpose(0, 2, 1).reshape(views * nt, ns)
+        assert rel_rmse(got, ref) < tol, algo
