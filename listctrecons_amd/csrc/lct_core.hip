@@ -50,12 +50,67 @@ int scratch(lct_ctx *ctx, int slot, size_t bytes, void **out)
     return LCT_OK;
 }
 
+static int staging_init(lct_ctx *ctx)
+{
+    lct_ctx::Staging &s = ctx->staging;
+    if (s.base || s.failed) return LCT_OK;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, 2 * lct_ctx::kStagingHalf, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        s.failed = true;
+        return LCT_OK;
+    }
+    for (int k = 0; k < 2; k++) {
+        hipError_t e = hipEventCreateWithFlags(&s.ev[k], hipEventDisableTiming);
+        if (e != hipSuccess) {
+            if (k == 1) (void)hipEventDestroy(s.ev[0]);
+            (void)hipHostFree(p);
+            return hip_error(e, "hipEventCreateWithFlags");
+        }
+    }
+    s.base = static_cast<char *>(p);
+    return LCT_OK;
+}
+
+void staging_release(lct_ctx *ctx)
+{
+    lct_ctx::Staging &s = ctx->staging;
+    if (!s.base) return;
+    for (int k = 0; k < 2; k++) {
+        if (s.pending[k]) (void)hipEventSynchronize(s.ev[k]);
+        (void)hipEventDestroy(s.ev[k]);
+    }
+    (void)hipHostFree(s.base);
+    s = lct_ctx::Staging{};
+}
+
 int upload(lct_ctx *ctx, void *d, const void *h, size_t bytes)
 {
     // Ordered after earlier work on the context stream (which may still read
     // the previous table in `d`); returns once the host buffer is consumed.
-    LCT_CHECK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
-    LCT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    if (bytes == 0) return LCT_OK;
+    LCT_CHECK(staging_init(ctx));
+    lct_ctx::Staging &s = ctx->staging;
+    if (!s.base || bytes > lct_ctx::kStagingHalf) {
+        // no staging ring, or a table larger than a half: copy and wait
+        LCT_CHECK_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, ctx->stream));
+        LCT_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+        return LCT_OK;
+    }
+    size_t off = (s.used + 255) & ~size_t(255);
+    if (off + bytes > lct_ctx::kStagingHalf) {
+        // switch halves; wait for the copies still reading the other one
+        s.half ^= 1;
+        if (s.pending[s.half]) LCT_CHECK_HIP(hipEventSynchronize(s.ev[s.half]));
+        s.pending[s.half] = false;
+        off = 0;
+    }
+    char *p = s.base + size_t(s.half) * lct_ctx::kStagingHalf + off;
+    std::memcpy(p, h, bytes);
+    s.used = off + bytes;
+    LCT_CHECK_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    LCT_CHECK_HIP(hipEventRecord(s.ev[s.half], ctx->stream));
+    s.pending[s.half] = true;
     return LCT_OK;
 }
 
@@ -130,6 +185,7 @@ int lct_ctx_destroy(lct_ctx *ctx)
     if (!ctx) return LCT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    staging_release(ctx);
     for (auto &b : ctx->scratch)
         if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->events_ready)

@@ -30,6 +30,21 @@ struct lct_ctx {
     hipEvent_t ev[8] = {};
     bool events_ready = false;
 
+    // Pinned staging ring for host tables (two halves of kStagingHalf bytes):
+    // an upload copies the table into the current half and enqueues an
+    // asynchronous copy from there, so the host never waits for the device.
+    // ev[k] follows the last copy out of half k; a half is refilled only once
+    // that event has completed.
+    static constexpr size_t kStagingHalf = size_t(8) << 20;
+    struct Staging {
+        char *base = nullptr;
+        size_t used = 0;
+        int half = 0;
+        bool pending[2] = {false, false};
+        hipEvent_t ev[2] = {};
+        bool failed = false;  // pinned allocation refused: uploads wait instead
+    } staging;
+
     // Where the last reconstruct call left its last block's filtered views
     // (scratch slots A / B, untouched until the next reconstruct call):
     // layout 1: rows [view][nt][ns] f32; layout 2: Q [view][cp][rp] of
@@ -62,8 +77,12 @@ void clear_error();
 // Returns a device buffer of at least `bytes` in slot `slot` (grown, never shrunk).
 int scratch(lct_ctx *ctx, int slot, size_t bytes, void **out);
 
-// Host-to-device copy on the context stream; returns after the copy is done.
+// Host-to-device copy on the context stream, ordered after earlier work on it;
+// returns once `h` has been read (into the pinned staging ring), without
+// waiting for the device.
 int upload(lct_ctx *ctx, void *d, const void *h, size_t bytes);
+// Waits for the staging ring's copies and frees it (context teardown).
+void staging_release(lct_ctx *ctx);
 
 // Kernel-side launchers (defined in the .hip files).
 int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *g, const float *d_fan,
