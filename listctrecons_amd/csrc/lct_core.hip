@@ -108,9 +108,14 @@ int upload(lct_ctx *ctx, void *d, const void *h, size_t bytes)
     char *p = s.base + size_t(s.half) * lct_ctx::kStagingHalf + off;
     std::memcpy(p, h, bytes);
     s.used = off + bytes;
+    // the half's event must follow every copy out of it: a copy on another
+    // stream than the half's previous one is ordered after that one first
+    if (s.pending[s.half] && s.ev_stream[s.half] != ctx->stream)
+        LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, s.ev[s.half], 0));
     LCT_CHECK_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
     LCT_CHECK_HIP(hipEventRecord(s.ev[s.half], ctx->stream));
     s.pending[s.half] = true;
+    s.ev_stream[s.half] = ctx->stream;
     return LCT_OK;
 }
 
@@ -185,6 +190,10 @@ int lct_ctx_destroy(lct_ctx *ctx)
     if (!ctx) return LCT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) {
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipStreamDestroy(ctx->aux);
+    }
     staging_release(ctx);
     for (auto &b : ctx->scratch)
         if (b.ptr) (void)hipFree(b.ptr);

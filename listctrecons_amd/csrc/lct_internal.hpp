@@ -17,6 +17,10 @@
 struct lct_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream (non-blocking, default priority), created on first use:
+    // the resident pipeline rebins and filters its later blocks' views on it
+    // while the first blocks are backprojected on `stream`.
+    hipStream_t aux = nullptr;
     bool profiling = false;
     lct_timings timings{};
 
@@ -42,6 +46,7 @@ struct lct_ctx {
         int half = 0;
         bool pending[2] = {false, false};
         hipEvent_t ev[2] = {};
+        hipStream_t ev_stream[2] = {};  // stream of the copy ev[k] follows
         bool failed = false;  // pinned allocation refused: uploads wait instead
     } staging;
 

@@ -35,6 +35,13 @@ long long count_updates(const lct_plan &plan);
 
 constexpr int kRowPad = 4;  // zero row slots before row 0
 constexpr int kTile = 16;   // 16 x 16 voxel columns per workgroup
+// Blocks backprojected on the context stream while the aux stream rebins and
+// filters the views of the rest (the whole range on one stream when it has
+// no more blocks than this).
+#ifndef LCT_LEAD_BLOCKS
+#define LCT_LEAD_BLOCKS 1
+#endif
+constexpr int kLeadBlocks = LCT_LEAD_BLOCKS;
 
 template <typename T> struct Cvt;
 template <> struct Cvt<float> {
@@ -602,27 +609,13 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         return LCT_OK;
     };
 
-    // 1. rebin every needed view once (fused pre-weighting).
-    LCT_CHECK(tic());
-    LCT_CHECK(launch_rebin(ctx, o.rebin_kind, o.weight_kind, &g, d_fan, 0, file_views, P,
-                           R.nviews, R.g0 + plan.data_offset, 0));
-    LCT_CHECK(toc(&t.rebin_ms));
-
-    // Per-view cos/sin (double) of the resident views, plus the block table.
-    const int n_cs = int(R.nviews + h + 1);
-    std::vector<float> cs_host(2 * size_t(n_cs));
-    {
-        std::vector<float> b(n_cs);
-        for (int i = 0; i < n_cs; i++) {
-            const long src = R.g0 + i;
-            b[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
-        }
-        angle_tables(b.data(), n_cs, cs_host);
-    }
-
     if (!fold) {
-        // Filter in place and backproject each block from its offset with the
-        // per-voxel kernel.
+        // Rebin every needed view once (fused pre-weighting), filter in place
+        // and backproject each block from its offset with the per-voxel kernel.
+        LCT_CHECK(tic());
+        LCT_CHECK(launch_rebin(ctx, o.rebin_kind, o.weight_kind, &g, d_fan, 0, file_views, P,
+                               R.nviews, R.g0 + plan.data_offset, 0));
+        LCT_CHECK(toc(&t.rebin_ms));
         LCT_CHECK(tic());
         LCT_CHECK(launch_filter_direct(ctx, g.ns, static_cast<const float *>(d_taps), P,
                                        long(R.nviews) * g.nt));
@@ -646,38 +639,59 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         return LCT_OK;
     }
 
-    // 2. filter into Q (backprojector layout, storage precision).
+    // Q: every view rebinned (fused pre-weighting) and filtered once, in the
+    // backprojector's layout and storage precision.
     const int cp = g.ns + 3, rp = g.nt + 8;
     const size_t esz = o.storage == LCT_F32 ? 4 : 2;
-    const size_t q_bytes = size_t(cp) * rp * esz * R.nviews;
-    LCT_CHECK(scratch(ctx, kScratchB, q_bytes, &d_q));
-    LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_bytes, ctx->stream));
-    LCT_CHECK(tic());
-    const long rows = long(R.nviews) * g.nt;
-    if (o.filter_algo == LCT_FILTER_TOEPLITZ_BF16X3 && g.ns % 4 == 0)
-        LCT_CHECK(launch_filter_toeplitz(ctx, o.storage, g.ns, g.nt, taps.data(), P, d_q, rows,
-                                         cp, rp));
-    else if (o.storage == LCT_F32)
-        LCT_CHECK(launch_filter_to_q<float>(ctx, g.ns, g.nt, static_cast<const float *>(d_taps),
-                                            P, static_cast<float *>(d_q), rows, cp, rp));
-    else if (o.storage == LCT_F16)
-        LCT_CHECK(launch_filter_to_q<__half>(ctx, g.ns, g.nt, static_cast<const float *>(d_taps),
-                                             P, static_cast<__half *>(d_q), rows, cp, rp));
-    else
-        LCT_CHECK(launch_filter_to_q<__hip_bfloat16>(ctx, g.ns, g.nt,
-                                                     static_cast<const float *>(d_taps), P,
-                                                     static_cast<__hip_bfloat16 *>(d_q), rows,
-                                                     cp, rp));
-    LCT_CHECK(toc(&t.filter_ms));
+    const size_t q_view = size_t(cp) * rp * esz;  // bytes of one view in Q
+    LCT_CHECK(scratch(ctx, kScratchB, q_view * R.nviews, &d_q));
+    LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_view * R.nviews, ctx->stream));
+    // rebin and filter of the resident views [v0, v1) on the context stream
+    auto prep = [&](long v0, long v1) -> int {
+        LCT_CHECK(tic());
+        LCT_CHECK(launch_rebin(ctx, o.rebin_kind, o.weight_kind, &g, d_fan, 0, file_views,
+                               P + size_t(v0) * view, v1 - v0, R.g0 + v0 + plan.data_offset, 0));
+        LCT_CHECK(toc(&t.rebin_ms));
+        const float *in = P + size_t(v0) * view;
+        void *q = static_cast<char *>(d_q) + size_t(v0) * q_view;
+        const long rows = (v1 - v0) * g.nt;
+        const float *dt = static_cast<const float *>(d_taps);
+        LCT_CHECK(tic());
+        if (o.filter_algo == LCT_FILTER_TOEPLITZ_BF16X3 && g.ns % 4 == 0)
+            LCT_CHECK(launch_filter_toeplitz(ctx, o.storage, g.ns, g.nt, taps.data(), in, q, rows,
+                                             cp, rp));
+        else if (o.storage == LCT_F32)
+            LCT_CHECK(launch_filter_to_q<float>(ctx, g.ns, g.nt, dt, in, static_cast<float *>(q),
+                                                rows, cp, rp));
+        else if (o.storage == LCT_F16)
+            LCT_CHECK(launch_filter_to_q<__half>(ctx, g.ns, g.nt, dt, in, static_cast<__half *>(q),
+                                                 rows, cp, rp));
+        else
+            LCT_CHECK(launch_filter_to_q<__hip_bfloat16>(ctx, g.ns, g.nt, dt, in,
+                                                         static_cast<__hip_bfloat16 *>(q), rows,
+                                                         cp, rp));
+        LCT_CHECK(toc(&t.filter_ms));
+        return LCT_OK;
+    };
 
-    // 3. backproject every block from Q in one launch.
+    // Per-view cos/sin (double) of the resident views, the block table and
+    // the slices whose window holds pair j (a_middle is the same in every block).
+    const int n_cs = int(R.nviews + h + 1);
+    std::vector<float> cs_host(2 * size_t(n_cs));
+    {
+        std::vector<float> b(n_cs);
+        for (int i = 0; i < n_cs; i++) {
+            const long src = R.g0 + i;
+            b[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
+        }
+        angle_tables(b.data(), n_cs, cs_host);
+    }
     std::vector<int> tab(34 * size_t(nb));
     for (int k = 0; k < nb; k++) {
         tab[34 * k] = int(long(R.bps[k]) + g.add_projections - R.g0);
         tab[34 * k + 1] = k * LCT_BLOCK_SLICES;
         for (int z = 0; z < 32; z++) tab[34 * k + 2 + z] = amid[z];
     }
-    // slices whose window holds pair j (a_middle is the same in every block)
     std::vector<unsigned> zmask(size_t(amid[31] - amid[0] + h));
     for (size_t jj = 0; jj < zmask.size(); jj++) {
         const int j = amid[0] - h + int(jj);
@@ -691,9 +705,6 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     float *d_cs = static_cast<float *>(d_tab);
     int *d_st = reinterpret_cast<int *>(d_cs + cs_host.size());
     unsigned *d_zm = reinterpret_cast<unsigned *>(d_st + tab.size());
-    LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
-    LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
-    LCT_CHECK(upload(ctx, d_zm, zmask.data(), zmask.size() * 4));
 
     FoldBpArgs a{};
     a.q = d_q;
@@ -705,7 +716,6 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.W = std::min(2 * a.H + 2, cp);
     a.cosb = d_cs;
     a.sinb = d_cs + n_cs;
-    a.slab_tab = d_st;
     a.zmask = d_zm;
     a.vol = d_vol;
     a.n_tiles_x = (ig.nx + kTile - 1) / kTile;
@@ -731,24 +741,75 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.ns_f = float(g.ns);
     a.nt_f = float(g.nt);
     a.scale = float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
-    LCT_CHECK(tic());
-    if (o.interp == LCT_INTERP_GATHER) {
-        if (o.storage == LCT_F32)
-            LCT_CHECK(launch_bp_fold_gather<float>(ctx, a, nb));
+    // backprojection of blocks [k0, k1) in one launch on the context stream
+    auto bp = [&](int k0, int k1) -> int {
+        FoldBpArgs b = a;
+        b.slab_tab = d_st + 34 * k0;
+        LCT_CHECK(tic());
+        if (o.interp == LCT_INTERP_GATHER) {
+            if (o.storage == LCT_F32)
+                LCT_CHECK(launch_bp_fold_gather<float>(ctx, b, k1 - k0));
+            else if (o.storage == LCT_F16)
+                LCT_CHECK(launch_bp_fold_gather<__half>(ctx, b, k1 - k0));
+            else
+                LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, b, k1 - k0));
+        } else if (o.storage == LCT_F32)
+            LCT_CHECK(launch_bp_fold<float>(ctx, b, k1 - k0));
         else if (o.storage == LCT_F16)
-            LCT_CHECK(launch_bp_fold_gather<__half>(ctx, a, nb));
+            LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0));
         else
-            LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, a, nb));
-    } else if (o.storage == LCT_F32)
-        LCT_CHECK(launch_bp_fold<float>(ctx, a, nb));
-    else if (o.storage == LCT_F16)
-        LCT_CHECK(launch_bp_fold<__half>(ctx, a, nb));
-    else
-        LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, a, nb));
-    LCT_CHECK(toc(&t.backproject_ms));
-    t.n_backproject_launches++;
+            LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, b, k1 - k0));
+        LCT_CHECK(toc(&t.backproject_ms));
+        t.n_backproject_launches++;
+        return LCT_OK;
+    };
+    auto upload_tables = [&]() -> int {
+        LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
+        LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
+        LCT_CHECK(upload(ctx, d_zm, zmask.data(), zmask.size() * 4));
+        return LCT_OK;
+    };
     ctx->last = lct_ctx::LastBlock{2, o.storage, g.n_angle, g.nt, g.ns, cp, rp,
                                    long(R.bps[nb - 1]) + g.add_projections - R.g0, d_q};
+
+    const int lead = std::min(nb, kLeadBlocks);
+    if (ctx->profiling || lead == nb) {
+        // one stream, stage after stage (the stage timers wait for each)
+        LCT_CHECK(prep(0, R.nviews));
+        LCT_CHECK(upload_tables());
+        return bp(0, nb);
+    }
+    // Two streams.  The views of the first `lead` blocks are rebinned and
+    // filtered, then those blocks are backprojected on the context stream
+    // while the remaining views are rebinned and filtered on the aux stream,
+    // which then backprojects the remaining blocks.  Both launches run
+    // together; the context stream waits for the aux stream at the end.
+    if (!ctx->aux) LCT_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+    const long v_lead = long(R.bps[lead - 1]) + g.add_projections + g.n_angle - R.g0;
+    hipEvent_t lead_done = ctx->ev[0], tables_done = ctx->ev[1], aux_done = ctx->ev[2];
+    LCT_CHECK(prep(0, v_lead));
+    LCT_CHECK_HIP(hipEventRecord(lead_done, ctx->stream));
+    LCT_CHECK(upload_tables());
+    LCT_CHECK_HIP(hipEventRecord(tables_done, ctx->stream));
+    LCT_CHECK(bp(0, lead));
+    const hipStream_t main = ctx->stream;
+    ctx->stream = ctx->aux;
+    int rc = [&]() -> int {
+        // the aux stream's rebin and filter tables reuse the scratch the
+        // lead stage's kernels read: wait for them first
+        LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, lead_done, 0));
+        LCT_CHECK(prep(v_lead, R.nviews));
+        LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, tables_done, 0));
+        LCT_CHECK(bp(lead, nb));
+        LCT_CHECK_HIP(hipEventRecord(aux_done, ctx->stream));
+        return LCT_OK;
+    }();
+    ctx->stream = main;
+    if (rc != LCT_OK) {
+        (void)hipStreamSynchronize(ctx->aux);
+        return rc;
+    }
+    LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, aux_done, 0));
     return LCT_OK;
 }
 

@@ -198,6 +198,40 @@ def test_slabs_equal_the_whole_volume(ctx, clinical, preset):
     assert e.value.code == _lib.LCT_ERR_RANGE
 
 
+@pytest.mark.parametrize("preset", ["fast_f32", "mixed_f16", "mixed_f16_gather"])
+def test_two_streams_equal_one_stream(ctx, clinical, preset):
+    """The resident pipeline backprojects block 0 on the context stream while
+    a second stream rebins and filters the other blocks' views.  The volume
+    equals, bit for bit, the one-stream run (profiling mode runs the stages
+    one after the other).  Also with the context on a non-default stream, and
+    with three volumes enqueued back to back before one wait."""
+    w, fan, a, c = clinical
+    print(f"two streams {preset}", flush=True)
+    pl = recon.plan_helical(w.prm(), a, c)
+    assert pl.n_blocks == 8
+    opts = recon.preset(preset, window=w.window)
+    ctx.set_profiling(True)
+    try:
+        serial = ctx.reconstruct(pl, opts, a, c, fan).clone()
+    finally:
+        ctx.set_profiling(False)
+    got = ctx.reconstruct(pl, opts, a, c, fan)
+    ctx.synchronize()
+    assert torch.equal(got, serial)
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    ctx.set_stream(s.cuda_stream)
+    try:
+        outs = [torch.empty_like(serial) for _ in range(3)]
+        for o in outs:
+            ctx.reconstruct(pl, opts, a, c, fan, o)
+        ctx.synchronize()
+    finally:
+        ctx.set_stream(0)
+    for o in outs:
+        assert torch.equal(o, serial)
+
+
 def test_large_detector_paths_agree(ctx):
     """1024^2 x 512 from 12288 views of 96 x 896 (16 blocks), in full: the
     pair-folded resident backprojector against the plain per-voxel path (whose
