@@ -29,7 +29,6 @@ namespace lct {
 
 constexpr int kC = 32;      // hidden channels
 constexpr int kTaps = 27;
-constexpr int kLayers = 6;
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
@@ -598,7 +597,6 @@ unsigned blocks_for(long n) { return unsigned((n + 255) / 256); }
 
 int forward(lct_cnn *n, const float *x, Shape s, float *y)
 {
-    const long nv = long(s.B) * s.D * s.H * s.W;
     hipStream_t st = n->ctx->stream;
     const long tiles = long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) *
                        ((s.W + kTX - 1) / kTX);

@@ -190,10 +190,6 @@ int lct_ctx_destroy(lct_ctx *ctx)
     if (!ctx) return LCT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    if (ctx->aux) {
-        (void)hipStreamSynchronize(ctx->aux);
-        (void)hipStreamDestroy(ctx->aux);
-    }
     staging_release(ctx);
     for (auto &b : ctx->scratch)
         if (b.ptr) (void)hipFree(b.ptr);

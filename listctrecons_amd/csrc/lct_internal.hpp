@@ -17,10 +17,6 @@
 struct lct_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Second stream (non-blocking, default priority), created on first use:
-    // the resident pipeline rebins and filters its later blocks' views on it
-    // while the first blocks are backprojected on `stream`.
-    hipStream_t aux = nullptr;
     bool profiling = false;
     lct_timings timings{};
 

@@ -35,13 +35,6 @@ long long count_updates(const lct_plan &plan);
 
 constexpr int kRowPad = 4;  // zero row slots before row 0
 constexpr int kTile = 16;   // 16 x 16 voxel columns per workgroup
-// Blocks backprojected on the context stream while the aux stream rebins and
-// filters the views of the rest (the whole range on one stream when it has
-// no more blocks than this).
-#ifndef LCT_LEAD_BLOCKS
-#define LCT_LEAD_BLOCKS 1
-#endif
-constexpr int kLeadBlocks = LCT_LEAD_BLOCKS;
 
 template <typename T> struct Cvt;
 template <> struct Cvt<float> {
@@ -150,11 +143,8 @@ struct FoldBpArgs {
     int lp;              // LDS row pitch of a staged channel (rp + 1: odd, so the
                          // channels that neighbouring lanes read fall in different banks)
     int asin_poly;       // every voxel column has |p_hat| / dso <= 0.7: asin_poly is valid
-    const float *cosb, *sinb;  // indexed by Q view
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
-    const unsigned *zmask;     // per pair j - (a_middle[0] - h): bit z set when
-                               // a_middle[z] - h <= j < a_middle[z] (the pair
-                               // lies in slice z's window); the same for every block
+    int n_pairs;               // pairs j of a block: a_middle[0] - h <= j < a_middle[31]
     float *vol;
     int n_tiles_x, n_tiles_y, n_wg, xcd_remap;
     int nx, ny, h;
@@ -226,8 +216,15 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // G (LCT_INTERP_GATHER): the same arithmetic with every tap gathered from Q
 // in HBM through the L1 / L2 caches instead: no staging, no barriers; the
 // zero guard slots of Q give the reference's border behaviour.
+// cs: (cos, sin) of each Q view's tube angle; zmask: per pair j - jlo, bit z
+// set when a_middle[z] - h <= j < a_middle[z] (the pair lies in slice z's
+// window; the same for every block); win: per (block, tile, pair) the first
+// channel slots of the two views' LDS windows (s1 | s2 << 16).  All three are
+// read with wave-uniform indices, so they arrive in scalar registers.
 template <typename T, int NPRE, int LP, bool G = false>
-__global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
+__global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
+                                                      const unsigned *__restrict__ zmask,
+                                                      const unsigned *__restrict__ win)
 {
     extern __shared__ float lds[];
     constexpr int CE = Chunk<T>::kElems;
@@ -256,8 +253,6 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
 
     const float xc = (float(ix) - A.wx) * A.dx;
     const float yc = (float(iy) - A.wy) * A.dy;
-    const float xcc = (float(tx0) + 0.5f * (kTile - 1) - A.wx) * A.dx;
-    const float ycc = (float(ty0) + 0.5f * (kTile - 1) - A.wy) * A.dy;
     const T *Q = static_cast<const T *>(A.q);
 
     float acc[32];
@@ -268,18 +263,13 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     const int jlo = st[2] - A.h, jhi = st[33];
 
     uint4 pre[NPRE];
-    // window of the pair whose first view has tube angle cos c, sin s
-    auto win_of = [&](float c, float s, int &s1, int &s2) {
-        const float ph = -xcc * c - ycc * s;
-        const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
-        s1 = window_start(c1, A.H, A.cp, W);
-        s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, W);
-    };
     // Chunk slot i of this thread: chunk index c = tid + 256 i, view
     // (c >= pad, the same on every lane of a wave), element offset in the
     // view's window and LDS offset, all fixed for the whole loop (a chunk
-    // never straddles a channel: rp is a multiple of 8).
-    int src_off[NPRE], dst[NPRE];
+    // never straddles a channel: rp is a multiple of 8).  Both offsets fit
+    // 16 bits (a window holds at most 2 * W * lp < 65536 floats) and share
+    // one register.
+    unsigned offs[NPRE];
     bool live[NPRE];
 #pragma unroll
     for (int i = 0; i < NPRE; i++) {
@@ -288,8 +278,7 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         const int m = c - v * pad;
         live[i] = m < n_vc;
         const int e = (live[i] ? m : 0) * CE;
-        src_off[i] = e;
-        dst[i] = v * lds_view + (e / rp) * lp + e % rp;
+        offs[i] = unsigned(e) | (unsigned(v * lds_view + (e / rp) * lp + e % rp) << 16);
     }
     auto fetch = [&](int j, int s1, int s2) {
         const T *va = Q + size_t(qoff + j) * A.view_stride + size_t(s1) * rp;
@@ -298,64 +287,36 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
         for (int i = 0; i < NPRE; i++) {
             const bool vbw = __builtin_amdgcn_readfirstlane(int(threadIdx.x) + i * 256) >= pad;
             const T *base = vbw ? vb : va;
-            if (live[i]) pre[i] = *reinterpret_cast<const uint4 *>(base + unsigned(src_off[i]));
+            if (live[i]) pre[i] = *reinterpret_cast<const uint4 *>(base + (offs[i] & 0xffffu));
         }
     };
     auto commit = [&](float *buf) {
 #pragma unroll
         for (int i = 0; i < NPRE; i++)
-            if (live[i]) chunk_to_lds_odd<T>(buf + dst[i], pre[i]);
+            if (live[i]) chunk_to_lds_odd<T>(buf + (offs[i] >> 16), pre[i]);
     };
 
-    // Per-pair scalars of 64 consecutive pairs [b0, b0 + 64), one pair per
-    // lane (cos/sin of both views, slice mask, both window starts); pair j
-    // reads them from lane j - b0.  Refilled every 64 pairs, so the uniform
-    // window asin and the table loads are not repeated on every lane.
-    const int lane = int(threadIdx.x) & 63;
-    const float *cosq = A.cosb + qoff, *sinq = A.sinb + qoff;
-    float bc, bs, bc2, bs2;
-    int bzm, bwin;
-    auto batch = [&](int b) {
-        const int jj = min(b + lane, jhi - 1);
-        bc = cosq[jj];
-        bs = sinq[jj];
-        bc2 = cosq[jj + A.h];
-        bs2 = sinq[jj + A.h];
-        bzm = int(A.zmask[jj - jlo]);
-        if constexpr (!G) {
-            int w1, w2;
-            win_of(bc, bs, w1, w2);
-            bwin = w1 | (w2 << 16);
-        } else {
-            bwin = 0;
-        }
-    };
-    auto lanef = [](float v, int l) {
-        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-    };
-    int b0 = jlo;
-    batch(b0);
+    const float2 *csq = cs + qoff;
+    const unsigned *wt = win + size_t(blk * tiles + t) * A.n_pairs;  // index j - jlo
     if constexpr (!G) {
-        const int w = __builtin_amdgcn_readlane(bwin, 0);
-        fetch(jlo, w & 0xffff, w >> 16);
+        const unsigned w = wt[0];
+        fetch(jlo, int(w & 0xffffu), int(w >> 16));
         commit(lds);
         __syncthreads();
     }
     for (int j = jlo; j < jhi; j++) {
-        const int l = j - b0;
-        const float c = lanef(bc, l), s = lanef(bs, l);
-        const float c2 = lanef(bc2, l), s2v = lanef(bs2, l);
-        const unsigned zm = unsigned(__builtin_amdgcn_readlane(bzm, l));
-        const int wj = __builtin_amdgcn_readlane(bwin, l);
-        const int s1 = wj & 0xffff, s2 = wj >> 16;
-        if (l == 63) {
-            b0 += 64;
-            batch(b0);
-        }
+        const float2 cs1 = csq[j], cs2 = csq[j + A.h];
+        const float c = cs1.x, s = cs1.y, c2 = cs2.x, s2v = cs2.y;
+        const unsigned zm = zmask[j - jlo];
+        int s1 = 0, s2 = 0;
         if constexpr (!G) {
+            const unsigned w = wt[j - jlo];
+            s1 = int(w & 0xffffu);
+            s2 = int(w >> 16);
             // prefetch pair j + 1 into registers (the last pair is fetched twice)
-            const int wn = __builtin_amdgcn_readlane(bwin, j + 1 - b0);
-            fetch(min(j + 1, jhi - 1), wn & 0xffff, wn >> 16);
+            const int jn = min(j + 1, jhi - 1);
+            const unsigned wn = wt[jn - jlo];
+            fetch(jn, int(wn & 0xffffu), int(wn >> 16));
         }
         const float *buf = lds;
         // z-independent terms of the pair (Backprojector.cu:53-62)
@@ -455,21 +416,55 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A)
     }
 }
 
+// First channel slots of the LDS windows of every (block, tile, pair): the
+// tile centre's channel coordinate through view j (first window) and its
+// mirror 2 ws - c through view j + h (second window).
+__global__ __launch_bounds__(256) void bp_windows_kernel(FoldBpArgs A, int n_blocks,
+                                                         const float2 *__restrict__ cs,
+                                                         unsigned *__restrict__ win)
+{
+    const int tiles = A.n_tiles_x * A.n_tiles_y;
+    const long n = long(n_blocks) * tiles * A.n_pairs;
+    const long i = long(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int jj = int(i % A.n_pairs);
+    const long bt = i / A.n_pairs;
+    const int blk = int(bt / tiles), t = int(bt % tiles);
+    const int tx0 = (t % A.n_tiles_x) * kTile, ty0 = (t / A.n_tiles_x) * kTile;
+    const float xcc = (float(tx0) + 0.5f * (kTile - 1) - A.wx) * A.dx;
+    const float ycc = (float(ty0) + 0.5f * (kTile - 1) - A.wy) * A.dy;
+    const int *st = A.slab_tab + blk * 34;
+    const int j = st[2] - A.h + jj;
+    const float2 c = cs[st[0] + j];
+    const float ph = -xcc * c.x - ycc * c.y;
+    const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
+    const int s1 = window_start(c1, A.H, A.cp, A.W);
+    const int s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, A.W);
+    win[i] = unsigned(s1) | (unsigned(s2) << 16);
+}
+
+// Device tables of the pair-folded backprojector (see bp_fold_kernel).
+struct BpTables {
+    const float2 *cs;
+    const unsigned *zmask;
+    const unsigned *win;  // unused by the gather path
+};
+
 template <typename T>
-static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
+static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     a.lp = a.rp + 1;
     a.W = 2;  // no LDS window
     hipLaunchKernelGGL((bp_fold_kernel<T, 1, 0, true>), dim3(unsigned(a.n_wg)), dim3(256), 0,
-                       ctx->stream, a);
+                       ctx->stream, a, tb.cs, tb.zmask, tb.win);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
 
 template <typename T>
-static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
+static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
@@ -485,7 +480,7 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     if (npre < 1 || npre > 8)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window too large");
     // LDS pitches of the 16-, 64- and 96-row detectors get immediate offsets
-    using Kern = void (*)(FoldBpArgs);
+    using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const unsigned *);
     static const Kern table[4][8] = {
 #define LCT_ROW(LP)                                                                              \
     {bp_fold_kernel<T, 1, LP>, bp_fold_kernel<T, 2, LP>, bp_fold_kernel<T, 3, LP>,               \
@@ -499,7 +494,8 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks)
     if (shm > 64 * 1024)
         LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
-    hipLaunchKernelGGL(k, dim3(unsigned(a.n_wg)), dim3(256), shm, ctx->stream, a);
+    hipLaunchKernelGGL(k, dim3(unsigned(a.n_wg)), dim3(256), shm, ctx->stream, a, tb.cs, tb.zmask,
+                       tb.win);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -684,7 +680,12 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
             const long src = R.g0 + i;
             b[i] = src < plan.n_readings ? h_angles[src] : 0.0f;
         }
-        angle_tables(b.data(), n_cs, cs_host);
+        std::vector<float> sep;
+        angle_tables(b.data(), n_cs, sep);  // [cos...][sin...] -> (cos, sin) pairs
+        for (int i = 0; i < n_cs; i++) {
+            cs_host[2 * size_t(i)] = sep[size_t(i)];
+            cs_host[2 * size_t(i) + 1] = sep[size_t(n_cs) + i];
+        }
     }
     std::vector<int> tab(34 * size_t(nb));
     for (int k = 0; k < nb; k++) {
@@ -702,9 +703,17 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     }
     const size_t tab_bytes = cs_host.size() * 4 + tab.size() * 4 + zmask.size() * 4;
     LCT_CHECK(scratch(ctx, kScratchTables, tab_bytes, &d_tab));
-    float *d_cs = static_cast<float *>(d_tab);
+    float *d_cs = static_cast<float *>(d_tab);  // 8-byte aligned: cs_host has 2 n_cs floats
     int *d_st = reinterpret_cast<int *>(d_cs + cs_host.size());
     unsigned *d_zm = reinterpret_cast<unsigned *>(d_st + tab.size());
+    const int n_pairs = int(zmask.size());
+    const int n_tiles = ((ig.nx + kTile - 1) / kTile) * ((ig.ny + kTile - 1) / kTile);
+    const long n_win = long(nb) * n_tiles * n_pairs;
+    void *d_win = nullptr;
+    if (o.interp != LCT_INTERP_GATHER)
+        LCT_CHECK(scratch(ctx, kScratchMisc0, size_t(n_win) * 4, &d_win));
+    const BpTables tb{reinterpret_cast<const float2 *>(d_cs), d_zm,
+                      static_cast<const unsigned *>(d_win)};
 
     FoldBpArgs a{};
     a.q = d_q;
@@ -714,9 +723,8 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.H = tile_half_width(g, ig);
     a.asin_poly = column_u_max(g, ig) <= 0.7 ? 1 : 0;
     a.W = std::min(2 * a.H + 2, cp);
-    a.cosb = d_cs;
-    a.sinb = d_cs + n_cs;
-    a.zmask = d_zm;
+    a.slab_tab = d_st;
+    a.n_pairs = n_pairs;
     a.vol = d_vol;
     a.n_tiles_x = (ig.nx + kTile - 1) / kTile;
     a.n_tiles_y = (ig.ny + kTile - 1) / kTile;
@@ -745,20 +753,22 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     auto bp = [&](int k0, int k1) -> int {
         FoldBpArgs b = a;
         b.slab_tab = d_st + 34 * k0;
+        BpTables tk = tb;  // the launch numbers its blocks from 0
+        if (tk.win) tk.win += size_t(k0) * n_tiles * n_pairs;
         LCT_CHECK(tic());
         if (o.interp == LCT_INTERP_GATHER) {
             if (o.storage == LCT_F32)
-                LCT_CHECK(launch_bp_fold_gather<float>(ctx, b, k1 - k0));
+                LCT_CHECK(launch_bp_fold_gather<float>(ctx, b, k1 - k0, tk));
             else if (o.storage == LCT_F16)
-                LCT_CHECK(launch_bp_fold_gather<__half>(ctx, b, k1 - k0));
+                LCT_CHECK(launch_bp_fold_gather<__half>(ctx, b, k1 - k0, tk));
             else
-                LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, b, k1 - k0));
+                LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, b, k1 - k0, tk));
         } else if (o.storage == LCT_F32)
-            LCT_CHECK(launch_bp_fold<float>(ctx, b, k1 - k0));
+            LCT_CHECK(launch_bp_fold<float>(ctx, b, k1 - k0, tk));
         else if (o.storage == LCT_F16)
-            LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0));
+            LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0, tk));
         else
-            LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, b, k1 - k0));
+            LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, b, k1 - k0, tk));
         LCT_CHECK(toc(&t.backproject_ms));
         t.n_backproject_launches++;
         return LCT_OK;
@@ -767,50 +777,21 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
         LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
         LCT_CHECK(upload(ctx, d_zm, zmask.data(), zmask.size() * 4));
+        if (d_win) {
+            if (n_win > 0x7fffffffL * 256L)
+                return set_error(LCT_ERR_UNSUPPORTED, "backprojector: window table too large");
+            hipLaunchKernelGGL(bp_windows_kernel, dim3(unsigned((n_win + 255) / 256)), dim3(256), 0,
+                               ctx->stream, a, nb, tb.cs, static_cast<unsigned *>(d_win));
+            LCT_CHECK_HIP(hipGetLastError());
+        }
         return LCT_OK;
     };
     ctx->last = lct_ctx::LastBlock{2, o.storage, g.n_angle, g.nt, g.ns, cp, rp,
                                    long(R.bps[nb - 1]) + g.add_projections - R.g0, d_q};
 
-    const int lead = std::min(nb, kLeadBlocks);
-    if (ctx->profiling || lead == nb) {
-        // one stream, stage after stage (the stage timers wait for each)
-        LCT_CHECK(prep(0, R.nviews));
-        LCT_CHECK(upload_tables());
-        return bp(0, nb);
-    }
-    // Two streams.  The views of the first `lead` blocks are rebinned and
-    // filtered, then those blocks are backprojected on the context stream
-    // while the remaining views are rebinned and filtered on the aux stream,
-    // which then backprojects the remaining blocks.  Both launches run
-    // together; the context stream waits for the aux stream at the end.
-    if (!ctx->aux) LCT_CHECK_HIP(hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
-    const long v_lead = long(R.bps[lead - 1]) + g.add_projections + g.n_angle - R.g0;
-    hipEvent_t lead_done = ctx->ev[0], tables_done = ctx->ev[1], aux_done = ctx->ev[2];
-    LCT_CHECK(prep(0, v_lead));
-    LCT_CHECK_HIP(hipEventRecord(lead_done, ctx->stream));
+    LCT_CHECK(prep(0, R.nviews));
     LCT_CHECK(upload_tables());
-    LCT_CHECK_HIP(hipEventRecord(tables_done, ctx->stream));
-    LCT_CHECK(bp(0, lead));
-    const hipStream_t main = ctx->stream;
-    ctx->stream = ctx->aux;
-    int rc = [&]() -> int {
-        // the aux stream's rebin and filter tables reuse the scratch the
-        // lead stage's kernels read: wait for them first
-        LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, lead_done, 0));
-        LCT_CHECK(prep(v_lead, R.nviews));
-        LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, tables_done, 0));
-        LCT_CHECK(bp(lead, nb));
-        LCT_CHECK_HIP(hipEventRecord(aux_done, ctx->stream));
-        return LCT_OK;
-    }();
-    ctx->stream = main;
-    if (rc != LCT_OK) {
-        (void)hipStreamSynchronize(ctx->aux);
-        return rc;
-    }
-    LCT_CHECK_HIP(hipStreamWaitEvent(ctx->stream, aux_done, 0));
-    return LCT_OK;
+    return bp(0, nb);
 }
 
 // Q[view0 + v][c + 1][r + kRowPad] -> rows[v][r][c] (f32).

@@ -199,14 +199,14 @@ def test_slabs_equal_the_whole_volume(ctx, clinical, preset):
 
 
 @pytest.mark.parametrize("preset", ["fast_f32", "mixed_f16", "mixed_f16_gather"])
-def test_two_streams_equal_one_stream(ctx, clinical, preset):
-    """The resident pipeline backprojects block 0 on the context stream while
-    a second stream rebins and filters the other blocks' views.  The volume
-    equals, bit for bit, the one-stream run (profiling mode runs the stages
-    one after the other).  Also with the context on a non-default stream, and
-    with three volumes enqueued back to back before one wait."""
+def test_back_to_back_volumes_equal_one_at_a_time(ctx, clinical, preset):
+    """Host tables reach the device through the context's pinned staging ring
+    without waits, so volumes can be enqueued back to back.  Three volumes
+    enqueued before one wait, with the context on a non-default stream, equal
+    bit for bit the volume of a profiled run (which waits after every
+    stage)."""
     w, fan, a, c = clinical
-    print(f"two streams {preset}", flush=True)
+    print(f"back to back {preset}", flush=True)
     pl = recon.plan_helical(w.prm(), a, c)
     assert pl.n_blocks == 8
     opts = recon.preset(preset, window=w.window)
