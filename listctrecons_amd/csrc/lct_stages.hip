@@ -20,52 +20,58 @@ namespace lct {
 
 // CUDA/Rebiner.cu:82-103 (FAN2PAR) and :13-29 (NULL, frac == 0).
 // par[ia][it][is] = lerp_view(fan[.][it][is], base + ia + k0[is] + fr[is]).
-__global__ __launch_bounds__(256) void rebin_view_lerp_kernel(
+// A thread computes V consecutive output views of one channel.  Output view
+// ia reads fan views v0 = base + ia + k0[is] and v0 + 1, so views
+// ia .. ia+V-1 read the V + 1 fan views v0 .. v0 + V, each loaded once per
+// row.
+template <int V>
+__global__ __launch_bounds__(256) void rebin_views_lerp_kernel(
     const float *__restrict__ fan, long fan_views, float *__restrict__ par, long base, int ns,
     int nt, const int *__restrict__ k0, const float *__restrict__ fr,
     const float *__restrict__ wtab, long n_out)
 {
-    // 1-D grid (a multiple of 8) of (channel chunk, output view) pairs; each
-    // XCD takes a contiguous range, so the consecutive output views that
-    // read the same fan views share its L2.
+    // 1-D grid (a multiple of 8) of (channel chunk, group of V output views);
+    // each XCD takes a contiguous range of groups (their fan views share its L2)
     const long n_wg = long(gridDim.x);
     const long wg = long(blockIdx.x & 7u) * (n_wg >> 3) + long(blockIdx.x >> 3);
     const long n_cx = (ns + 255) / 256;
-    const long ia = wg / n_cx;
-    const int is = int(wg - ia * n_cx) * 256 + int(threadIdx.x);
-    if (ia >= n_out || is >= ns) return;
-    const long v0 = base + ia + k0[is];
+    const long grp = wg / n_cx;
+    const int is = int(wg - grp * n_cx) * 256 + int(threadIdx.x);
+    const long ia0 = grp * V;
+    if (ia0 >= n_out || is >= ns) return;
+    const int nv = int(n_out - ia0 < V ? n_out - ia0 : V);
+    const long v0 = base + ia0 + k0[is];
     const float f = fr[is];
-    const bool ok0 = v0 >= 0 && v0 < fan_views;
-    const bool ok1 = v0 + 1 >= 0 && v0 + 1 < fan_views;
     const size_t view = size_t(nt) * ns;
-    const float *a = fan + (ok0 ? size_t(v0) * view : 0) + is;
-    const float *b = fan + (ok1 ? size_t(v0 + 1) * view : 0) + is;
-    float *o = par + size_t(ia) * view + is;
-    // 8 rows per step: 16 independent loads in flight per thread
-    int it = 0;
-    for (; it + 8 <= nt; it += 8) {
-        float x[8], y[8];
+    const float *src = fan + is;
+    bool ok[V + 1];
+    size_t off[V + 1];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            x[u] = ok0 ? a[size_t(it + u) * ns] : 0.0f;
-            y[u] = ok1 ? b[size_t(it + u) * ns] : 0.0f;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            float r = fmaf(f, y[u] - x[u], x[u]);
-            if (wtab) r *= wtab[size_t(it + u) * ns + is];
-            o[size_t(it + u) * ns] = r;
-        }
+    for (int j = 0; j <= V; j++) {
+        ok[j] = v0 + j >= 0 && v0 + j < fan_views;
+        off[j] = ok[j] ? size_t(v0 + j) * view : 0;
     }
-    for (; it < nt; it++) {
-        const float x = ok0 ? a[size_t(it) * ns] : 0.0f;
-        const float y = ok1 ? b[size_t(it) * ns] : 0.0f;
-        float r = fmaf(f, y - x, x);
-        if (wtab) r *= wtab[size_t(it) * ns + is];
-        o[size_t(it) * ns] = r;
+    float *o = par + size_t(ia0) * view + is;
+#pragma unroll 4
+    for (int it = 0; it < nt; it++) {
+        const size_t r = size_t(it) * ns;
+        float x[V + 1];
+#pragma unroll
+        for (int j = 0; j <= V; j++) x[j] = ok[j] ? src[off[j] + r] : 0.0f;
+        const float w = wtab ? wtab[r + is] : 1.0f;
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            if (j >= nv) break;
+            float v = fmaf(f, x[j + 1] - x[j], x[j]);
+            if (wtab) v *= w;
+            o[size_t(j) * view + r] = v;
+        }
     }
 }
+
+// Output views per thread of the rebin (clinical scan: 1.24 ms with 1,
+// 0.99 with 2, 0.82 with 4, 0.71 with 8, 0.72 with 16).
+constexpr int kRebinViews = 8;
 
 // CUDA/Rebiner.cu:156-179 (PI_ORI): trilinear sample at (pI, qI, bI).
 __global__ __launch_bounds__(256) void rebin_pi_ori_kernel(
@@ -193,9 +199,11 @@ int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *gp, cons
                            d_par, out_view0, ns, nt, d_i, d_f, d_i + ns, d_f + ns, d_f + 2 * ns,
                            d_f + 3 * ns, g.wt, d_w);
     } else {
-        const long n_wg = ((long((ns + 255) / 256) * n_out) + 7) & ~7L;
+        constexpr int V = kRebinViews;
+        const long n_grp = (n_out + V - 1) / V;
+        const long n_wg = ((long((ns + 255) / 256) * n_grp) + 7) & ~7L;
         if (n_wg > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "rebin: too many views");
-        hipLaunchKernelGGL(rebin_view_lerp_kernel, dim3(unsigned(n_wg)), dim3(256), 0,
+        hipLaunchKernelGGL(rebin_views_lerp_kernel<V>, dim3(unsigned(n_wg)), dim3(256), 0,
                            ctx->stream, fan, fan_views, d_par, out_view0, ns, nt, d_i, d_f, d_w,
                            long(n_out));
     }
