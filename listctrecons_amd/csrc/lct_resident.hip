@@ -265,10 +265,13 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A, const float2
     uint4 pre[NPRE];
     // Chunk slot i of this thread: chunk index c = tid + 256 i, view
     // (c >= pad, the same on every lane of a wave), element offset in the
-    // view's window and LDS offset, all fixed for the whole loop (a chunk
-    // never straddles a channel: rp is a multiple of 8).  Both offsets fit
-    // 16 bits (a window holds at most 2 * W * lp < 65536 floats) and share
-    // one register.
+    // view's window and LDS offset, all fixed for the whole loop.  Chunk m
+    // of a view holds CE consecutive rows of one channel (rp is a multiple
+    // of 8): channel m % W, row block m / W.  Consecutive lanes thus take
+    // consecutive channels, whose LDS addresses differ by the odd pitch lp
+    // and fall in different banks when the chunk is written.  Both offsets
+    // fit 16 bits (a window holds at most 2 * W * lp < 65536 floats) and
+    // share one register.
     unsigned offs[NPRE];
     bool live[NPRE];
 #pragma unroll
@@ -277,8 +280,9 @@ __global__ __launch_bounds__(256) void bp_fold_kernel(FoldBpArgs A, const float2
         const int v = c >= pad ? 1 : 0;
         const int m = c - v * pad;
         live[i] = m < n_vc;
-        const int e = (live[i] ? m : 0) * CE;
-        offs[i] = unsigned(e) | (unsigned(v * lds_view + (e / rp) * lp + e % rp) << 16);
+        const int mm = live[i] ? m : 0;
+        const int ch = mm % W, rb = (mm / W) * CE;
+        offs[i] = unsigned(ch * rp + rb) | (unsigned(v * lds_view + ch * lp + rb) << 16);
     }
     auto fetch = [&](int j, int s1, int s2) {
         const T *va = Q + size_t(qoff + j) * A.view_stride + size_t(s1) * rp;
