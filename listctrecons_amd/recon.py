@@ -172,10 +172,11 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter exit the HIP runtime may already be torn down: the
-        # process releases device memory itself
-        if sys.is_finalizing():
+        # process releases device memory itself (the default argument keeps
+        # the check reachable after module globals are cleared)
+        if _finalizing():
             return
         try:
             self.close()
@@ -395,10 +396,11 @@ class Cnn:
             lib().lct_cnn_destroy(self.h)
             self.h = None
 
-    def __del__(self):
+    def __del__(self, _finalizing=sys.is_finalizing):
         # at interpreter exit the HIP runtime may already be torn down: the
-        # process releases device memory itself
-        if sys.is_finalizing():
+        # process releases device memory itself (the default argument keeps
+        # the check reachable after module globals are cleared)
+        if _finalizing():
             return
         try:
             self.close()
