@@ -14,6 +14,11 @@
  *    allocation on the context's device); h_* arguments are host pointers.
  *  - Work is stream-ordered on the context's stream (the null stream unless
  *    lct_ctx_set_stream was called) and asynchronous; lct_synchronize waits.
+ *    Host arrays passed to a call (angles, couch, taps, parameters) are
+ *    copied into the context's pinned staging ring (16 MB) before the call
+ *    returns, so the caller may reuse them at once; the copies reach the
+ *    device in stream order.  Device buffers must stay valid until the
+ *    stream has finished with them.
  *  - Array layouts follow the reference: projections [view][row][channel],
  *    volumes [z][y][x], all row-major, channel / x fastest.
  */
