@@ -221,6 +221,41 @@ def stream_slabs(ctx, pl, opts, a, c, host_fan, vol, blocks_per_slab, reps=2):
     return best, len(slabs), max_views, uploaded[0]
 
 
+def cli_end_to_end(w, host_fan, head_vol, path):
+    """The drop-in CLI (FDKReconstructor through liblct) on the reference's files:
+    writes proj.raw, angles.raw, couch.raw and scan.prm to a scratch directory,
+    runs `LISTCTRecons scan.prm <factory string>` and times the process (reading
+    the files, HIP initialisation, reconstruction, writing image.raw and
+    rebin.raw).  Its image must equal the library's volume."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    from listctrecons_amd import build, scan
+    words = {"mixed_f16": "f16", "mixed_bf16": "bf16", "fast_f32": "resident"}
+    if path not in words:
+        raise SystemExit(f"--cli-e2e: no factory string for path {path}")
+    window = {1: "hann-", 2: "shepp-"}.get(int(w.window), "")
+    rtype = f"Helicalfdkramprebingpu-{window}{words[path]}"
+    d = tempfile.mkdtemp(prefix="lct_cli_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        prm = scan.write_scan_files(w, d, host_fan.numpy())
+        t0 = time.perf_counter()
+        r = subprocess.run([build.CLI, prm, rtype], stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, text=True)
+        dt = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit(f"CLI failed ({r.returncode}): {r.stdout[-2000:]}")
+        img = np.fromfile(os.path.join(d, "image.raw"), np.float32).reshape(head_vol.shape)
+        return {"s_per_volume": round(dt, 3), "factory": rtype,
+                "identical_to_library": bool(np.array_equal(img, head_vol)),
+                "note": "one LISTCTRecons process on files in a scratch directory: file "
+                        "reads, HIP initialisation, reconstruction and image writes; the "
+                        "files were just written, so they may be read from the page cache"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def load_profile(kernel_tag):
     """Per-launch counter figures of a kernel on a workload, committed under
     profiles/ (rocprofv3 runs); {} when there are none."""
@@ -270,6 +305,9 @@ def main():
     ap.add_argument("--stream-slabs", type=int, default=2,
                     help="also time slab streaming from pinned host memory, N blocks per slab "
                          "(0: off)")
+    ap.add_argument("--cli-e2e", action="store_true",
+                    help="also time the LISTCTRecons CLI on the workload's files (file I/O, "
+                         "process start and HIP initialisation included)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one device copy of this many GiB first (PMC scale reference)")
     args = ap.parse_args()
@@ -346,6 +384,11 @@ def main():
                             "reconstructed"}
         log(f"  {st_s*1e3:.1f} ms/volume streamed in {n_slabs} slabs")
         del svol
+
+    cli = None
+    if args.cli_e2e:
+        cli = cli_end_to_end(w, host_fan, head_vol, args.path)
+        log(f"  CLI end to end {cli['s_per_volume']:.3f} s/volume")
     del host_fan
 
     plain = None
@@ -457,6 +500,8 @@ def main():
     }
     if streamed:
         out["slab_streaming"] = streamed
+    if cli:
+        out["cli_end_to_end"] = cli
     if variants:
         out["variants"] = variants
     if learned:
