@@ -227,8 +227,12 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // window; the same for every block); win: per (block, tile, pair) the first
 // channel slots of the two views' LDS windows (s1 | s2 << 16).  All three are
 // read with wave-uniform indices, so they arrive in scalar registers.
+// f32 storage stages 4 chunks per thread and would use 84 VGPRs (2 workgroups
+// per CU); capped at 80 it keeps 3 (4 spilled, 37.7 -> 34.9 ms on the
+// clinical scan).  16-bit storage needs 71 and no cap.
 template <typename T, int NPRE, int LP, bool G = false>
-__global__ __launch_bounds__(kBpThreads) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
+__global__ __launch_bounds__(kBpThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
                                                       const unsigned *__restrict__ zmask,
                                                       const unsigned *__restrict__ win)
 {
