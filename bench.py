@@ -9,7 +9,8 @@ the mixed-precision one (f16 filtered projections, FP32 accumulation); bf16
 storage and the plain FP32 path (the reference's per-block loop on the GPU)
 are timed beside it, and the FP64 oracle is timed on the host on a bounded
 sample.  Optional: slab streaming from pinned host memory (--stream-slabs),
-the learned correction's inference (--no-learned to skip).
+the learned correction's inference (--no-learned to skip), the LISTCTRecons
+CLI end to end on the workload's files (--cli-e2e).
 
 Prints one JSON line (rank 0).
 """
