@@ -377,9 +377,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
                 // selects instead of divergent branches (emits no instruction)
                 asm volatile("" : "+v"(wb));
                 // conjugate weights (w1, 1 - w1); both rays off the detector: 1/2 each
-                const float w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : wb);
-                const float r1 = __builtin_amdgcn_fmed3f(qh + row0, lo_r, hi_r);
-                const float r2 = __builtin_amdgcn_fmed3f(qh2 + row0, lo_r, hi_r);
+                // when every lane has both rays on the detector (a wave-uniform
+                // test on the compare masks), the row coordinates lie in the
+                // detector's slots and w1 = wb: the clamps and selects are skipped
+                float w1 = wb, r1 = qh + row0, r2 = qh2 + row0;
+                if (__builtin_amdgcn_ballot_w64(out1 || out2) != 0) {
+                    // keeps this a branch: not speculated into selects (no instruction)
+                    asm volatile("");
+                    w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : w1);
+                    r1 = __builtin_amdgcn_fmed3f(r1, lo_r, hi_r);
+                    r2 = __builtin_amdgcn_fmed3f(r2, lo_r, hi_r);
+                }
                 const int i1 = int(r1), i2 = int(r2);
                 const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
                 float a00, a01, a10, a11, c00, c01, c10, c11;
