@@ -46,6 +46,7 @@ struct ToeplitzArgs {
     void *q;                     // Q [v][cp][rp]
     long n_cols;                 // V * nt
     int ns, nt, cp, rp, kp, n_m_tiles, n_n_wg;
+    int xcd_remap;  // grid is a multiple of 8: deal consecutive tiles to one XCD
 };
 
 __device__ __forceinline__ unsigned short bf16_rn(float x)
@@ -73,8 +74,13 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int mt = blockIdx.x % A.n_m_tiles;
-    const int nw = blockIdx.x / A.n_m_tiles;
+    // workgroups are dealt round-robin to the 8 XCDs; the remap gives each XCD
+    // a contiguous range, so the channel tiles that read the same columns share
+    // that XCD's L2 instead of fetching them once per XCD
+    int b = int(blockIdx.x);
+    if (A.xcd_remap) b = (b & 7) * int(gridDim.x >> 3) + (b >> 3);
+    const int mt = b % A.n_m_tiles;
+    const int nw = b / A.n_m_tiles;
     const int m0 = mt * kTM;
     const int li = lane & 15;
     const int lk = (lane >> 4) * 8;
@@ -245,6 +251,10 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
     a.n_m_tiles = mp / kTM;
     const long n_tiles = (n_cols + kTN - 1) / kTN;
     a.n_n_wg = int(std::min<long>(n_tiles, std::max<long>(1, 4096 / a.n_m_tiles)));
+    // a grid that is a multiple of 8 takes the XCD-aware order (fewer strided
+    // workgroups still cover every column tile)
+    while (a.n_n_wg > 8 && (a.n_m_tiles * a.n_n_wg) % 8 != 0) a.n_n_wg--;
+    a.xcd_remap = (a.n_m_tiles * a.n_n_wg) % 8 == 0 ? 1 : 0;
     hipLaunchKernelGGL(toeplitz_filter_kernel<T>, dim3(unsigned(a.n_m_tiles * a.n_n_wg)), dim3(256),
                        0, ctx->stream, a);
     LCT_CHECK_HIP(hipGetLastError());
