@@ -7,7 +7,8 @@ Outputs (git-ignored, shipped with the tree):
   listctrecons_amd/libListCTRecons.so   C++ drop-in classes over the C ABI
   listctrecons_amd/bin/LISTCTRecons     the reference CLI (LISTCTRecons <prm>)
   oracle/liblct_oracle.so               FP64 CPU oracle (test infrastructure)
-  oracle/_ref/*                         reference sources that compile alone
+  oracle/_ref/*                         reference sources that compile alone, and the
+                                        reference's main.cpp linked to the package
 """
 from __future__ import annotations
 
@@ -122,11 +123,21 @@ def build_native_tests() -> None:
         os.replace(exe + ".tmp", exe)
 
 
+def build_ref_main() -> None:
+    """The reference's Src/main.cpp compiled unchanged against the drop-in
+    headers and linked to the package (oracle/_ref/ref_main), when the
+    reference sources are present."""
+    oracle_dir = os.path.join(ROOT, "oracle")
+    if os.path.isdir("/root/reference"):
+        _run(["make", "-B", "-s", "-C", oracle_dir, "-f", "ref.mk", "_ref/ref_main"])
+
+
 def build_all(jobs: int = 8) -> None:
     build_oracle()
     build_lct(jobs)
     build_cpp(jobs)
     build_native_tests()
+    build_ref_main()
 
 
 if __name__ == "__main__":

@@ -140,3 +140,35 @@ def test_cli_refuses_a_foreign_checkpoint(tmp_path):
         r = run(CLI, prm, "Helicalfdkramprebingpu-resident-ckpt")
         assert r.returncode == 1
         assert "belongs to another reconstruction" in r.stdout
+
+
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "ref_main")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN), reason="reference main.cpp not built")
+def test_reference_main_binds_unchanged(files, tmp_path):
+    """The reference's own Src/main.cpp, compiled unchanged against
+    include/listctrecons and linked to libListCTRecons + liblct in place of its
+    CUDA objects (oracle/ref.mk), exits like this package's CLI: usage without
+    arguments, the parser's message and exit 1 on a missing key or file."""
+    w, prm = files
+    ref, ours = run(REF_MAIN), run(CLI)
+    assert ref.returncode == ours.returncode == 1
+    assert ref.stdout == ours.stdout == "Usage LISTCTRecons <PRM filename>\n"
+    text = "".join(l for l in open(prm) if not l.startswith("NProjTurn:"))
+    p = str(tmp_path / "missing.prm")
+    open(p, "w").write(text)
+    ref, ours = run(REF_MAIN, p), run(CLI, p)
+    assert ref.returncode == ours.returncode == 1
+    assert "Parameter not found in prm file:NProjTurn" in ref.stdout
+    assert ref.stdout == ours.stdout
+    ref, ours = run(REF_MAIN, str(tmp_path / "nope.prm")), run(CLI, str(tmp_path / "nope.prm"))
+    assert ref.returncode == ours.returncode == 1
+    assert ref.stdout == ours.stdout
+    # a scan too short for the planner: the same message from ReconData
+    ws = scan.workload("tiny", readings=300)
+    short = scan.write_scan_files(ws, str(tmp_path / "short"))
+    ref, ours = run(REF_MAIN, short), run(CLI, short)
+    assert ref.returncode == ours.returncode == 1
+    assert "outside of allowed data range" in ref.stdout
+    assert ref.stdout == ours.stdout

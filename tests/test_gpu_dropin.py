@@ -143,3 +143,31 @@ def test_gpu_memory_ctrl_semantics():
     r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=120)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout[-2000:]
+
+
+REF_MAIN = os.path.join(ROOT, "oracle", "_ref", "ref_main")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN), reason="reference main.cpp not built")
+def test_reference_main_reconstructs_like_the_cli(case):
+    """The reference's own Src/main.cpp, compiled unchanged against the drop-in
+    headers and linked to this package (oracle/ref.mk), runs its hard-coded
+    "Helicalfdkramprebingpu" reconstruction: image.raw and rebin.raw equal
+    this package's CLI with the same string byte for byte, and the image is
+    the FP64 oracle's Ram-Lak reconstruction to 1e-4."""
+    w, d, prm, fan, pl, a, c, ref = case
+    out = {}
+    for name, cmd in (("reference main", [REF_MAIN, prm]),
+                      ("cli", [CLI, prm, "Helicalfdkramprebingpu"])):
+        for f in ("image.raw", "rebin.raw"):
+            if os.path.exists(os.path.join(d, f)):
+                os.remove(os.path.join(d, f))
+        print(name, flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:]
+        out[name] = tuple(open(os.path.join(d, f), "rb").read() for f in ("image.raw", "rebin.raw"))
+    assert out["reference main"] == out["cli"]
+    img = np.frombuffer(out["cli"][0], np.float32).reshape(ref.shape)
+    ref_ramlak = O.reconstruct(pl, fan, a, c, window=0)
+    assert rel_rmse(img, ref_ramlak) <= 1e-4
