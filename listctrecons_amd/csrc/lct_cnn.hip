@@ -14,9 +14,14 @@
 // the same kernel with the taps mirrored and W transposed per tap, and the
 // 32 -> 1 output layer is the same kernel with one N fragment whose column 0
 // holds the (bf16) output weights.  The 1 -> 32 layer is an MFMA GEMM over
-// the 27 taps; all weight gradients run on FP32 VALU.
+// the 27 taps.  The 32 -> 32 weight gradients are implicit GEMMs over voxels
+// on the same MFMA; the weight gradients of the 1-channel layers and the bias
+// gradients run on FP32 VALU.  Every reduction over voxels goes through
+// per-chunk partial sums added in a fixed order (no float atomics), so the
+// loss and the gradients are bit-reproducible.
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -316,27 +321,63 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 }
 
 // ------------------------------------------------------------ backward
+//
+// Every reduction over voxels (loss, weight and bias gradients) is split into
+// a fixed number of chunks that depends only on the shape: each chunk writes
+// its partial sums to its own row of a [chunks][params] buffer, and
+// reduce_partials_kernel adds the rows in chunk order.  No float atomics: a
+// loss / gradient evaluation is bit-reproducible, run after run.
 
-// loss = mean((y - t)^2); dy = 2 (y - t) / N
-__global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ y,
-                                                  const float *__restrict__ t,
-                                                  float *__restrict__ dy, long n,
-                                                  float *__restrict__ loss)
+constexpr int kMaxChunks = 256;
+
+__device__ __forceinline__ void chunk_range(long n, int chunk, int n_chunks, long &v0, long &v1)
 {
-    __shared__ float red[256];
-    const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    float d = 0.0f;
-    if (i < n) {
-        d = y[i] - t[i];
-        dy[i] = 2.0f * d / float(n);
-    }
-    red[threadIdx.x] = d * d;
+    v0 = n * chunk / n_chunks;
+    v1 = n * (chunk + 1) / n_chunks;
+}
+
+// Sum of a 256-thread block's values in a fixed tree order (result on thread 0).
+__device__ __forceinline__ float block_sum(float x, float *red)
+{
+    red[threadIdx.x] = x;
     __syncthreads();
     for (int k = 128; k > 0; k >>= 1) {
         if (int(threadIdx.x) < k) red[threadIdx.x] += red[threadIdx.x + k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] / float(n));
+    return red[0];
+}
+
+// out[p] = scale * sum_c part[c][p], c ascending.
+__global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__restrict__ part,
+                                                              int n_chunks, long np,
+                                                              float *__restrict__ out, float scale)
+{
+    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    float acc = 0.0f;
+    for (int c = 0; c < n_chunks; c++) acc += part[size_t(c) * np + p];
+    out[p] = acc * scale;
+}
+
+// loss = mean((y - t)^2); dy = 2 (y - t) / N.  Chunk c (one block) writes
+// its sum of squares to part[c].
+__global__ __launch_bounds__(256) void mse_kernel(const float *__restrict__ y,
+                                                  const float *__restrict__ t,
+                                                  float *__restrict__ dy, long n, int n_chunks,
+                                                  float *__restrict__ part)
+{
+    __shared__ float red[256];
+    long v0, v1;
+    chunk_range(n, blockIdx.x, n_chunks, v0, v1);
+    float acc = 0.0f;
+    for (long i = v0 + threadIdx.x; i < v1; i += 256) {
+        const float d = y[i] - t[i];
+        dy[i] = 2.0f * d / float(n);
+        acc = fmaf(d, d, acc);
+    }
+    const float sum = block_sum(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
 // dh5[v][ci] = (h5[v][ci] > 0) * sum_tap w6[tap][ci] dy[v - off(tap)]
@@ -383,74 +424,163 @@ __global__ __launch_bounds__(256) void dgrad_out1_kernel(const float *__restrict
     }
 }
 
-// Weight gradient of a 32 -> 32 layer for one tap over a chunk of voxels:
-// dw[tap][co][ci] += sum_v g[v][co] a[v + off(tap)][ci]  (FP32 accumulation).
-// Thread t owns co = t / 8 and ci = 4 (t % 8) .. +3; voxels are staged in LDS
-// 128 at a time.
-constexpr int kWgChunk = 128;
+// Weight gradient of a 32 -> 32 layer on MFMA:
+//   dw[tap][co][ci] = sum_v g[v][co] a[v + off(tap)][ci]
+// is 27 GEMMs of 32 x 32 with K = voxels, on v_mfma_f32_16x16x32_bf16
+// (A = g^T, B = the shifted a; bf16 operands, FP32 accumulation).  A
+// workgroup owns one half of ci (blockIdx.y) and a contiguous range of the
+// conv tiles (4 z x 4 y x 16 x voxels, K = 256 per tile, 8 k-steps of 32);
+// its partial sums stay in registers across the tiles and go to row
+// blockIdx.x of the partial buffer at the end.  K must be contiguous per
+// lane in both operands, so the tiles are staged transposed in LDS:
+//   gT[co][v]                  v = (z * 4 + y) * 16 + x of the tile
+//   aC[dx + 1][ci][hz][hy][x]  = a(x0 + x + dx, y0 + hy - 1, z0 + hz - 1)
+// one copy of the ci half's halo per x shift, so that every tap's B fragment
+// (8 consecutive x) is one aligned 16-byte read.  Wave w owns the taps
+// w, w + 4, ... (7 or 6) and both co fragments: 56 accumulator registers.
+// Lane (q = lane / 16) covers k = 8 q .. 8 q + 7 of a k-step s: voxel row
+// 2 s + q / 2 of the tile, x = 8 (q % 2) .. + 7.  The pitches are odd
+// multiples of 16 bytes, so the 16 lanes of a fragment row read 16-byte
+// words in different bank groups.
+constexpr int kGP = kTZ * kTY * kTX + 8;  // 264 bf16 per co row of gT (528 B = 33 x 16 B)
+constexpr int kAP = kHZ * kHY * kTX + 8;  // 584 bf16 per ci row of aC (1168 B = 73 x 16 B)
+constexpr int kHaloItems = kHalo * 2;     // halo voxels x 2 chunks of 8 channels (one ci half)
+
 __global__ __launch_bounds__(256) void wgrad_mid_kernel(const bf16_t *__restrict__ g,
                                                         const bf16_t *__restrict__ a,
-                                                        float *__restrict__ dw, Shape s,
-                                                        long vox_per_wg)
+                                                        float *__restrict__ part, Shape s,
+                                                        int n_tiles, int n_chunks)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t sg[kWgChunk * kC];
-    __shared__ __attribute__((aligned(16))) bf16_t sa[kWgChunk * kC];
-    const int tap = blockIdx.y;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const long n = long(s.B) * s.D * s.H * s.W;
-    const long v_begin = long(blockIdx.x) * vox_per_wg;
-    const long v_end = (v_begin + vox_per_wg < n ? v_begin + vox_per_wg : n);
-    const int co = threadIdx.x >> 3, ci0 = (threadIdx.x & 7) * 4;
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (long vb = v_begin; vb < v_end; vb += kWgChunk) {
-        // stage: 128 voxels x 4 chunks of 16 B for g and for the shifted a
-        for (int c = threadIdx.x; c < kWgChunk * 4; c += 256) {
-            const long v = vb + (c >> 2);
-            const int part = c & 3;
-            uint4 gv = make_uint4(0u, 0u, 0u, 0u), av = make_uint4(0u, 0u, 0u, 0u);
-            if (v < v_end) {
-                gv = reinterpret_cast<const uint4 *>(g + v * kC)[part];
-                const int xx = int(v % s.W), yy = int((v / s.W) % s.H),
-                          zz = int((v / (long(s.W) * s.H)) % s.D);
-                const int b = int(v / (long(s.W) * s.H * s.D));
-                if (inside(s, zz + dz, yy + dy, xx + dx))
-                    av = reinterpret_cast<const uint4 *>(a + vox(s, b, zz + dz, yy + dy, xx + dx) * kC)[part];
-            }
-            reinterpret_cast<uint4 *>(sg)[c] = gv;
-            reinterpret_cast<uint4 *>(sa)[c] = av;
-        }
-        __syncthreads();
-        const int nv = int((v_end - vb < kWgChunk ? v_end - vb : kWgChunk));
-        for (int i = 0; i < nv; i++) {
-            const float gg = bf2f(sg[i * kC + co]);
-            const uint2 q = *reinterpret_cast<const uint2 *>(sa + i * kC + ci0);
-            acc[0] = fmaf(gg, bf2f(bf16_t(q.x & 0xffffu)), acc[0]);
-            acc[1] = fmaf(gg, bf2f(bf16_t(q.x >> 16)), acc[1]);
-            acc[2] = fmaf(gg, bf2f(bf16_t(q.y & 0xffffu)), acc[2]);
-            acc[3] = fmaf(gg, bf2f(bf16_t(q.y >> 16)), acc[3]);
-        }
-        __syncthreads();
-    }
-    float *d = dw + (size_t(tap) * kC + co) * kC + ci0;
+    __shared__ __attribute__((aligned(16))) bf16_t gT[kC * kGP];      // 16.5 KB
+    __shared__ __attribute__((aligned(16))) bf16_t aC[3 * 16 * kAP];  // 54.8 KB
+    const int chunk = blockIdx.x, half = blockIdx.y;
+    const int t_begin = int(long(n_tiles) * chunk / n_chunks);
+    const int t_end = int(long(n_tiles) * (chunk + 1) / n_chunks);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, q = lane >> 4;
+    const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
+    const int tiles_z = (s.D + kTZ - 1) / kTZ;
+
+    f32x4_t acc[7][2];
 #pragma unroll
-    for (int k = 0; k < 4; k++) atomicAdd(d + k, acc[k]);
+    for (int i = 0; i < 7; i++)
+#pragma unroll
+        for (int f = 0; f < 2; f++) acc[i][f] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int tile = t_begin; tile < t_end; tile++) {
+        long t = tile;
+        const int tx = int(t % tiles_x);
+        t /= tiles_x;
+        const int ty = int(t % tiles_y);
+        t /= tiles_y;
+        const int tz = int(t % tiles_z);
+        const int b = int(t / tiles_z);
+        const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+
+        // loads first (one memory latency), then the transposing LDS writes
+        uint4 gv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int c = int(threadIdx.x) + 256 * i;  // < 1024
+            const int vl = c >> 2, part_ = c & 3;
+            const int gz = z0 + (vl >> 6), gy = y0 + ((vl >> 4) & 3), gx = x0 + (vl & 15);
+            gv[i] = inside(s, gz, gy, gx)
+                        ? reinterpret_cast<const uint4 *>(g + vox(s, b, gz, gy, gx) * kC)[part_]
+                        : make_uint4(0u, 0u, 0u, 0u);
+        }
+        constexpr int kAS = (kHaloItems + 255) / 256;
+        uint4 av[kAS];
+#pragma unroll
+        for (int i = 0; i < kAS; i++) {
+            const int c = min(int(threadIdx.x) + 256 * i, kHaloItems - 1);
+            const int hv = c >> 1, part_ = c & 1;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+            av[i] = inside(s, gz, gy, gx)
+                        ? reinterpret_cast<const uint4 *>(a + vox(s, b, gz, gy, gx) * kC + half * 16)[part_]
+                        : make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();  // the previous tile's fragment reads are done
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int c = int(threadIdx.x) + 256 * i;
+            const int vl = c >> 2, part_ = c & 3;
+            const unsigned w4[4] = {gv[i].x, gv[i].y, gv[i].z, gv[i].w};
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                gT[(part_ * 8 + e) * kGP + vl] = bf16_t(e & 1 ? w4[e >> 1] >> 16 : w4[e >> 1] & 0xffffu);
+        }
+#pragma unroll
+        for (int i = 0; i < kAS; i++) {
+            const int c = int(threadIdx.x) + 256 * i;
+            if (c >= kHaloItems) break;
+            const int hv = c >> 1, part_ = c & 1;
+            const int hx = hv % kHX, hyz = hv / kHX;  // hyz = hz * 6 + hy
+            const unsigned w4[4] = {av[i].x, av[i].y, av[i].z, av[i].w};
+#pragma unroll
+            for (int dxi = 0; dxi < 3; dxi++) {
+                const int x = hx - dxi;
+                if (x < 0 || x >= kTX) continue;
+                bf16_t *d = aC + (dxi * 16 + part_ * 8) * kAP + hyz * kTX + x;
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    d[e * kAP] = bf16_t(e & 1 ? w4[e >> 1] >> 16 : w4[e >> 1] & 0xffffu);
+            }
+        }
+        __syncthreads();
+
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++) {
+            const int row = 2 * ks + (q >> 1), x8 = (q & 1) * 8;
+            const int zl = row >> 2, yl = row & 3;
+            bf16x8_t af[2];
+#pragma unroll
+            for (int f = 0; f < 2; f++)
+                af[f] = *reinterpret_cast<const bf16x8_t *>(gT + (f * 16 + li) * kGP + row * kTX + x8);
+#pragma unroll
+            for (int i = 0; i < 7; i++) {
+                const int tap = wave + 4 * i;
+                if (tap >= kTaps) break;  // wave-uniform
+                const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+                const bf16x8_t bfr = *reinterpret_cast<const bf16x8_t *>(
+                    aC + ((dx + 1) * 16 + li) * kAP + ((zl + dz + 1) * kHY + yl + dy + 1) * kTX + x8);
+#pragma unroll
+                for (int f = 0; f < 2; f++)
+                    acc[i][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[f], bfr, acc[i][f], 0, 0, 0);
+            }
+        }
+    }
+    // C fragment: lane holds rows (co) 4 q + r, column (ci) li
+    float *dst = part + size_t(chunk) * kTaps * kC * kC;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        const int tap = wave + 4 * i;
+        if (tap >= kTaps) break;
+#pragma unroll
+        for (int f = 0; f < 2; f++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                dst[(size_t(tap) * kC + f * 16 + 4 * q + r) * kC + half * 16 + li] = acc[i][f][r];
+    }
 }
 
 // Weight gradient of layer 1 (ci = x, one channel) and of layer 6 (co = y, one
-// channel): dw[tap][c] += sum_v G[v][c] A[v + off(tap)] (layer 1, G = g1, A = x)
-// or sum_v dy[v] h5[v + off(tap)][c] (layer 6).  Thread = (channel, voxel lane).
+// channel): dw[tap][c] = sum_v G[v][c] A[v + off(tap)] (layer 1, G = g1, A = x)
+// or sum_v dy[v] h5[v + off(tap)][c] (layer 6), on FP32 VALU (1/32 of a
+// 32 -> 32 layer's work).  Block (chunk, tap), thread = (channel, voxel lane);
+// part[chunk][tap][c].
 template <bool LAYER6>
 __global__ __launch_bounds__(256) void wgrad_io_kernel(const bf16_t *__restrict__ hid,
                                                        const float *__restrict__ one,
-                                                       float *__restrict__ dw, Shape s,
-                                                       long vox_per_wg)
+                                                       float *__restrict__ part, Shape s,
+                                                       int n_chunks)
 {
     __shared__ float red[256];
     const int tap = blockIdx.y;
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
     const long n = long(s.B) * s.D * s.H * s.W;
-    const long v_begin = long(blockIdx.x) * vox_per_wg;
-    const long v_end = (v_begin + vox_per_wg < n ? v_begin + vox_per_wg : n);
+    long v_begin, v_end;
+    chunk_range(n, blockIdx.x, n_chunks, v_begin, v_end);
     const int c = threadIdx.x & 31, lane_v = threadIdx.x >> 5;  // 8 voxel lanes
     float acc = 0.0f;
     for (long v = v_begin + lane_v; v < v_end; v += 8) {
@@ -468,27 +598,36 @@ __global__ __launch_bounds__(256) void wgrad_io_kernel(const bf16_t *__restrict_
     if (threadIdx.x < 32) {
         float t = 0.0f;
         for (int k = 0; k < 8; k++) t += red[k * 32 + threadIdx.x];
-        atomicAdd(dw + tap * kC + threadIdx.x, t);
+        part[(size_t(blockIdx.x) * kTaps + tap) * kC + threadIdx.x] = t;
     }
 }
 
-// Bias gradients: db[c] += sum_v g[v][c] (bf16 channels-last, C = 32) or sum_v dy[v].
+// Bias gradients: db[c] = sum_v g[v][c] (bf16 channels-last, C = 32; part
+// [chunk][32]) or sum_v g1[v] (part[chunk]).
 __global__ __launch_bounds__(256) void bgrad_kernel(const bf16_t *__restrict__ g,
                                                     const float *__restrict__ g1,
-                                                    float *__restrict__ db, long n)
+                                                    float *__restrict__ part, long n,
+                                                    int n_chunks)
 {
     __shared__ float red[256];
+    long v0, v1;
+    chunk_range(n, blockIdx.x, n_chunks, v0, v1);
+    if (!g) {
+        float acc = 0.0f;
+        for (long v = v0 + threadIdx.x; v < v1; v += 256) acc += g1[v];
+        const float sum = block_sum(acc, red);
+        if (threadIdx.x == 0) part[blockIdx.x] = sum;
+        return;
+    }
     const int c = threadIdx.x & 31, lane_v = threadIdx.x >> 5;
     float acc = 0.0f;
-    const long stride = long(gridDim.x) * 8;
-    for (long v = long(blockIdx.x) * 8 + lane_v; v < n; v += stride)
-        acc += g ? bf2f(g[v * kC + c]) : (c == 0 ? g1[v] : 0.0f);
+    for (long v = v0 + lane_v; v < v1; v += 8) acc += bf2f(g[v * kC + c]);
     red[threadIdx.x] = acc;
     __syncthreads();
     if (threadIdx.x < 32) {
         float t = 0.0f;
         for (int k = 0; k < 8; k++) t += red[k * 32 + threadIdx.x];
-        if (g || threadIdx.x == 0) atomicAdd(db + threadIdx.x, t);
+        part[size_t(blockIdx.x) * kC + threadIdx.x] = t;
     }
 }
 
@@ -560,6 +699,7 @@ struct lct_cnn {
     bf16_t *act[6] = {};   // h1..h5 (index 1..5)
     bf16_t *grad[2] = {};  // ping-pong gradient activations
     float *dy = nullptr, *y = nullptr, *loss = nullptr;
+    float *part = nullptr;  // [kMaxChunks][27 x 32 x 32] partial sums of the reductions
     long cap_vox = 0;
     long step = 0;
 };
@@ -613,38 +753,58 @@ int forward(lct_cnn *n, const float *x, Shape s, float *y)
     return LCT_OK;
 }
 
+// Voxel chunks of the reductions: a function of the shape only, so the
+// summation order (and every bit of the result) is the same on every run.
+int voxel_chunks(long nv) { return int(std::min<long>(kMaxChunks, std::max<long>(1, (nv + 255) / 256))); }
+long conv_tiles(Shape s)
+{
+    return long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) * ((s.W + kTX - 1) / kTX);
+}
+
+void reduce_into(lct_cnn *n, int n_chunks, long np, float *out)
+{
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(np)), dim3(256), 0, n->ctx->stream,
+                       n->part, n_chunks, np, out, 1.0f);
+}
+
 int backward(lct_cnn *n, const float *x, Shape s)
 {
     const long nv = long(s.B) * s.D * s.H * s.W;
     hipStream_t st = n->ctx->stream;
-    const long vpw = 8192;
-    const unsigned chunks = unsigned((nv + vpw - 1) / vpw);
+    const int vc = voxel_chunks(nv);
+    const long tiles = conv_tiles(s);
+    if (tiles > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "cnn: too many voxel tiles");
+    const int tc = int(std::min<long>(kMaxChunks, tiles));
     // layer 6
-    hipLaunchKernelGGL(wgrad_io_kernel<true>, dim3(chunks, kTaps), dim3(256), 0, st, n->act[5],
-                       n->dy, n->g + kW6, s, vpw);
-    hipLaunchKernelGGL(bgrad_kernel, dim3(256), dim3(256), 0, st, nullptr, n->dy, n->g + kB6, nv);
+    hipLaunchKernelGGL(wgrad_io_kernel<true>, dim3(vc, kTaps), dim3(256), 0, st, n->act[5], n->dy,
+                       n->part, s, vc);
+    reduce_into(n, vc, kTaps * kC, n->g + kW6);
+    hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, nullptr, n->dy, n->part, nv, vc);
+    reduce_into(n, vc, 1, n->g + kB6);
     hipLaunchKernelGGL(dgrad_out1_kernel, dim3(blocks_for(nv)), dim3(256), 0, st, n->dy,
                        n->wio + kTaps * kC, n->act[5], n->grad[0], s);
-    const long tiles = long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) *
-                       ((s.W + kTX - 1) / kTX);
     int cur = 0;
     // layers 5..2: grad[cur] holds dL/d(pre-activation of layer l+2) (masked)
     for (int l = 3; l >= 0; l--) {
         bf16_t *gk = n->grad[cur];
-        hipLaunchKernelGGL(wgrad_mid_kernel, dim3(chunks, kTaps), dim3(256), 0, st, gk,
-                           n->act[l + 1], n->g + kMid + l * kMidStride, s, vpw);
-        hipLaunchKernelGGL(bgrad_kernel, dim3(256), dim3(256), 0, st, gk, nullptr,
-                           n->g + kMid + l * kMidStride + kTaps * kC * kC, nv);
+        float *gw = n->g + kMid + l * kMidStride;
+        hipLaunchKernelGGL(wgrad_mid_kernel, dim3(tc, 2), dim3(256), 0, st, gk, n->act[l + 1],
+                           n->part, s, int(tiles), tc);
+        reduce_into(n, tc, long(kTaps) * kC * kC, gw);
+        hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, gk, nullptr, n->part, nv, vc);
+        reduce_into(n, vc, kC, gw + kTaps * kC * kC);
         hipLaunchKernelGGL(conv_mid_kernel<1>, dim3(unsigned(tiles)), dim3(256), 0, st, gk,
                            n->wt + size_t(l) * kTaps * kC * kC, nullptr, n->act[l + 1],
                            n->grad[cur ^ 1], s, nullptr, nullptr);
         cur ^= 1;
     }
     // layer 1: grad[cur] = dL/d(pre-activation of layer 1)
-    hipLaunchKernelGGL(wgrad_io_kernel<false>, dim3(chunks, kTaps), dim3(256), 0, st,
-                       n->grad[cur], x, n->g + kW1, s, vpw);
-    hipLaunchKernelGGL(bgrad_kernel, dim3(256), dim3(256), 0, st, n->grad[cur], nullptr,
-                       n->g + kB1, nv);
+    hipLaunchKernelGGL(wgrad_io_kernel<false>, dim3(vc, kTaps), dim3(256), 0, st, n->grad[cur], x,
+                       n->part, s, vc);
+    reduce_into(n, vc, kTaps * kC, n->g + kW1);
+    hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, n->grad[cur], nullptr, n->part, nv,
+                       vc);
+    reduce_into(n, vc, kC, n->g + kB1);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
@@ -683,6 +843,8 @@ int lct_cnn_create(lct_ctx *ctx, uint64_t seed, lct_cnn **out)
         return fail(e, "hipMalloc(w6op)");
     if ((e = hipMalloc(&n->w1op, long(kC) * kC * 2)) != hipSuccess) return fail(e, "hipMalloc(w1op)");
     if ((e = hipMalloc(&n->loss, 4)) != hipSuccess) return fail(e, "hipMalloc(loss)");
+    if ((e = hipMalloc(&n->part, size_t(kMaxChunks) * kTaps * kC * kC * 4)) != hipSuccess)
+        return fail(e, "hipMalloc(part)");
     // He-normal initialisation from the seed; the last layer starts at 1/100 so
     // the initial correction is close to zero; zero biases.
     std::vector<float> h(kParams, 0.0f);
@@ -715,7 +877,7 @@ int lct_cnn_destroy(lct_cnn *n)
                     static_cast<void *>(n->g), static_cast<void *>(n->wb), static_cast<void *>(n->wt),
                     static_cast<void *>(n->wio), static_cast<void *>(n->w6op),
                     static_cast<void *>(n->w1op),
-                    static_cast<void *>(n->loss),
+                    static_cast<void *>(n->loss), static_cast<void *>(n->part),
                     static_cast<void *>(n->dy), static_cast<void *>(n->y)})
         if (p) (void)hipFree(p);
     for (auto p : n->act) if (p) (void)hipFree(p);
@@ -766,10 +928,13 @@ int lct_cnn_loss_grad(lct_cnn *n, const float *d_x, const float *d_t, int B, int
     LCT_CHECK(cnn_alloc(n, nv));
     const Shape s{B, D, H, W};
     LCT_CHECK(forward(n, d_x, s, n->y));
-    LCT_CHECK_HIP(hipMemsetAsync(n->loss, 0, 4, n->ctx->stream));
-    LCT_CHECK_HIP(hipMemsetAsync(n->g, 0, kParams * 4, n->ctx->stream));
-    hipLaunchKernelGGL(mse_kernel, dim3(blocks_for(nv)), dim3(256), 0, n->ctx->stream, n->y, d_t,
-                       n->dy, nv, n->loss);
+    // every gradient entry and the loss are written (not accumulated) by the
+    // fixed-order reductions below
+    const int vc = voxel_chunks(nv);
+    hipLaunchKernelGGL(mse_kernel, dim3(vc), dim3(256), 0, n->ctx->stream, n->y, d_t, n->dy, nv, vc,
+                       n->part);
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, n->ctx->stream, n->part, vc, 1L,
+                       n->loss, 1.0f / float(nv));
     LCT_CHECK(backward(n, d_x, s));
     if (h_loss) {
         LCT_CHECK_HIP(hipMemcpyAsync(h_loss, n->loss, 4, hipMemcpyDeviceToHost, n->ctx->stream));
