@@ -348,16 +348,29 @@ __device__ __forceinline__ float block_sum(float x, float *red)
     return red[0];
 }
 
-// out[p] = scale * sum_c part[c][p], c ascending.
+// out[p] = scale * sum_c part[c][p] in a fixed order: a block of 256
+// threads covers P consecutive p (P = 64, 32 or 1) and 256 / P chunk groups;
+// group k sums the chunks c = k, k + 256/P, ... ascending, and the groups'
+// sums are added in group order.
 __global__ __launch_bounds__(256) void reduce_partials_kernel(const float *__restrict__ part,
-                                                              int n_chunks, long np,
+                                                              int n_chunks, long np, int P,
                                                               float *__restrict__ out, float scale)
 {
-    const long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
+    __shared__ float red[256];
+    const int pl = int(threadIdx.x) % P, grp = int(threadIdx.x) / P, n_grp = 256 / P;
+    const long p = long(blockIdx.x) * P + pl;
     float acc = 0.0f;
-    for (int c = 0; c < n_chunks; c++) acc += part[size_t(c) * np + p];
-    out[p] = acc * scale;
+    if (p < np) {
+#pragma unroll 8
+        for (int c = grp; c < n_chunks; c += n_grp) acc += part[size_t(c) * np + p];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (int(threadIdx.x) < P && p < np) {
+        float sum = 0.0f;
+        for (int k = 0; k < n_grp; k++) sum += red[k * P + threadIdx.x];
+        out[p] = sum * scale;
+    }
 }
 
 // loss = mean((y - t)^2); dy = 2 (y - t) / N.  Chunk c (one block) writes
@@ -564,52 +577,117 @@ __global__ __launch_bounds__(256) void wgrad_mid_kernel(const bf16_t *__restrict
     }
 }
 
-// Weight gradient of layer 1 (ci = x, one channel) and of layer 6 (co = y, one
-// channel): dw[tap][c] = sum_v G[v][c] A[v + off(tap)] (layer 1, G = g1, A = x)
-// or sum_v dy[v] h5[v + off(tap)][c] (layer 6), on FP32 VALU (1/32 of a
-// 32 -> 32 layer's work).  Block (chunk, tap), thread = (channel, voxel lane);
-// part[chunk][tap][c].
+// Weight gradient of layer 1 (ci = x, one channel) and of layer 6 (co = y,
+// one channel), on FP32 VALU (1/32 of a 32 -> 32 layer's work):
+//   layer 1: dw[tap][c] = sum_v g1[v][c] x[v + off(tap)]
+//   layer 6: dw[tap][c] = sum_v dy[v] h5[v + off(tap)][c]
+// Workgroup = a contiguous range of the conv tiles (4 z x 4 y x 16 x); per
+// tile the unshifted field is staged in LDS at the tile's voxels and the
+// shifted one on the (4+2) x (4+2) x (16+2) halo.  Thread t owns channel
+// t % 32 and the taps t / 32 + 8 k; its sums stay in registers across the
+// tiles and go to part[chunk][tap][c].
 template <bool LAYER6>
 __global__ __launch_bounds__(256) void wgrad_io_kernel(const bf16_t *__restrict__ hid,
                                                        const float *__restrict__ one,
                                                        float *__restrict__ part, Shape s,
-                                                       int n_chunks)
+                                                       int n_tiles, int n_chunks)
 {
-    __shared__ float red[256];
-    const int tap = blockIdx.y;
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const long n = long(s.B) * s.D * s.H * s.W;
-    long v_begin, v_end;
-    chunk_range(n, blockIdx.x, n_chunks, v_begin, v_end);
-    const int c = threadIdx.x & 31, lane_v = threadIdx.x >> 5;  // 8 voxel lanes
-    float acc = 0.0f;
-    for (long v = v_begin + lane_v; v < v_end; v += 8) {
-        const int xx = int(v % s.W), yy = int((v / s.W) % s.H), zz = int((v / (long(s.W) * s.H)) % s.D);
-        const int b = int(v / (long(s.W) * s.H * s.D));
-        if (!inside(s, zz + dz, yy + dy, xx + dx)) continue;
-        const long vs = vox(s, b, zz + dz, yy + dy, xx + dx);
-        if (LAYER6)
-            acc = fmaf(one[v], bf2f(hid[vs * kC + c]), acc);
-        else
-            acc = fmaf(bf2f(hid[v * kC + c]), one[vs], acc);
+    // layer 1: tile of g1 as f32 [256][32] + halo of x [648]
+    // layer 6: tile of dy [256] + halo of h5 as bf16 [648][32]
+    constexpr int kTileVox = kTZ * kTY * kTX;
+    constexpr int kBytes = LAYER6 ? kTileVox * 4 + kHalo * kC * 2 : kTileVox * kC * 4 + kHalo * 4;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kBytes];
+    float *tile_f = reinterpret_cast<float *>(lds);                       // [256][32] or [256]
+    float *halo_f = reinterpret_cast<float *>(lds + kTileVox * kC * 4);    // layer 1: [648]
+    bf16_t *halo_h = reinterpret_cast<bf16_t *>(lds + kTileVox * 4);       // layer 6: [648][32]
+    const int chunk = blockIdx.x;
+    const int t_begin = int(long(n_tiles) * chunk / n_chunks);
+    const int t_end = int(long(n_tiles) * (chunk + 1) / n_chunks);
+    const int c = threadIdx.x & 31, tg = threadIdx.x >> 5;
+    const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
+    const int tiles_z = (s.D + kTZ - 1) / kTZ;
+    int toff[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int tap = min(tg + 8 * k, kTaps - 1);
+        toff[k] = ((tap / 9 - 1) * kHY + (tap / 3) % 3 - 1) * kHX + tap % 3 - 1;
     }
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x < 32) {
-        float t = 0.0f;
-        for (int k = 0; k < 8; k++) t += red[k * 32 + threadIdx.x];
-        part[(size_t(blockIdx.x) * kTaps + tap) * kC + threadIdx.x] = t;
+    const int nk = tg + 24 < kTaps ? 4 : 3;
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int tile = t_begin; tile < t_end; tile++) {
+        long t = tile;
+        const int tx = int(t % tiles_x);
+        t /= tiles_x;
+        const int ty = int(t % tiles_y);
+        t /= tiles_y;
+        const int tz = int(t % tiles_z);
+        const int b = int(t / tiles_z);
+        const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+        __syncthreads();  // the previous tile's reads are done
+        if constexpr (!LAYER6) {
+            for (int i = threadIdx.x; i < kTileVox * 4; i += 256) {
+                const int vl = i >> 2, pt = i & 3;
+                const int gz = z0 + (vl >> 6), gy = y0 + ((vl >> 4) & 3), gx = x0 + (vl & 15);
+                uint4 u = make_uint4(0u, 0u, 0u, 0u);
+                if (inside(s, gz, gy, gx)) u = reinterpret_cast<const uint4 *>(hid + vox(s, b, gz, gy, gx) * kC)[pt];
+                const unsigned w4[4] = {u.x, u.y, u.z, u.w};
+                float *d = tile_f + vl * kC + pt * 8;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    d[2 * e] = bf2f(bf16_t(w4[e] & 0xffffu));
+                    d[2 * e + 1] = bf2f(bf16_t(w4[e] >> 16));
+                }
+            }
+            for (int i = threadIdx.x; i < kHalo; i += 256) {
+                const int hx = i % kHX, hy = (i / kHX) % kHY, hz = i / (kHX * kHY);
+                const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+                halo_f[i] = inside(s, gz, gy, gx) ? one[vox(s, b, gz, gy, gx)] : 0.0f;
+            }
+        } else {
+            for (int i = threadIdx.x; i < kTileVox; i += 256) {
+                const int gz = z0 + (i >> 6), gy = y0 + ((i >> 4) & 3), gx = x0 + (i & 15);
+                tile_f[i] = inside(s, gz, gy, gx) ? one[vox(s, b, gz, gy, gx)] : 0.0f;
+            }
+            for (int i = threadIdx.x; i < kHalo * 4; i += 256) {
+                const int hv = i >> 2, pt = i & 3;
+                const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+                const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+                uint4 u = make_uint4(0u, 0u, 0u, 0u);
+                if (inside(s, gz, gy, gx)) u = reinterpret_cast<const uint4 *>(hid + vox(s, b, gz, gy, gx) * kC)[pt];
+                reinterpret_cast<uint4 *>(halo_h)[i] = u;
+            }
+        }
+        __syncthreads();
+        for (int vl = 0; vl < kTileVox; vl++) {
+            const int h = (((vl >> 6) + 1) * kHY + ((vl >> 4) & 3) + 1) * kHX + (vl & 15) + 1;
+            if constexpr (!LAYER6) {
+                const float gv = tile_f[vl * kC + c];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k < nk) acc[k] = fmaf(gv, halo_f[h + toff[k]], acc[k]);
+            } else {
+                const float dv = tile_f[vl];
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k < nk) acc[k] = fmaf(dv, bf2f(halo_h[(h + toff[k]) * kC + c]), acc[k]);
+            }
+        }
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (k < nk) part[(size_t(chunk) * kTaps + tg + 8 * k) * kC + c] = acc[k];
 }
 
 // Bias gradients: db[c] = sum_v g[v][c] (bf16 channels-last, C = 32; part
-// [chunk][32]) or sum_v g1[v] (part[chunk]).
+// [chunk][32]) or sum_v g1[v] (part[chunk]).  Channels-last rows are read as
+// 16-byte chunks: thread t takes chunk t % 4 (channels 8 (t % 4) ..) of every
+// 64th voxel of the chunk's range.
 __global__ __launch_bounds__(256) void bgrad_kernel(const bf16_t *__restrict__ g,
                                                     const float *__restrict__ g1,
                                                     float *__restrict__ part, long n,
                                                     int n_chunks)
 {
-    __shared__ float red[256];
+    __shared__ float red[256 * 8];
     long v0, v1;
     chunk_range(n, blockIdx.x, n_chunks, v0, v1);
     if (!g) {
@@ -619,15 +697,25 @@ __global__ __launch_bounds__(256) void bgrad_kernel(const bf16_t *__restrict__ g
         if (threadIdx.x == 0) part[blockIdx.x] = sum;
         return;
     }
-    const int c = threadIdx.x & 31, lane_v = threadIdx.x >> 5;
-    float acc = 0.0f;
-    for (long v = v0 + lane_v; v < v1; v += 8) acc += bf2f(g[v * kC + c]);
-    red[threadIdx.x] = acc;
+    const int pt = threadIdx.x & 3, lane_v = threadIdx.x >> 2;
+    float acc[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (long v = v0 + lane_v; v < v1; v += 64) {
+        const uint4 u = reinterpret_cast<const uint4 *>(g + v * kC)[pt];
+        const unsigned w4[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            acc[2 * e] += bf2f(bf16_t(w4[e] & 0xffffu));
+            acc[2 * e + 1] += bf2f(bf16_t(w4[e] >> 16));
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[threadIdx.x * 8 + e] = acc[e];
     __syncthreads();
     if (threadIdx.x < 32) {
+        const int ch = threadIdx.x, p = ch >> 3, e = ch & 7;
         float t = 0.0f;
-        for (int k = 0; k < 8; k++) t += red[k * 32 + threadIdx.x];
-        part[size_t(blockIdx.x) * kC + threadIdx.x] = t;
+        for (int l = 0; l < 64; l++) t += red[(l * 4 + p) * 8 + e];
+        part[size_t(blockIdx.x) * kC + ch] = t;
     }
 }
 
@@ -761,10 +849,11 @@ long conv_tiles(Shape s)
     return long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) * ((s.W + kTX - 1) / kTX);
 }
 
-void reduce_into(lct_cnn *n, int n_chunks, long np, float *out)
+void reduce_into(lct_cnn *n, int n_chunks, long np, float *out, float scale = 1.0f)
 {
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(blocks_for(np)), dim3(256), 0, n->ctx->stream,
-                       n->part, n_chunks, np, out, 1.0f);
+    const int P = np >= 64 ? 64 : np >= 32 ? 32 : 1;
+    hipLaunchKernelGGL(reduce_partials_kernel, dim3(unsigned((np + P - 1) / P)), dim3(256), 0,
+                       n->ctx->stream, n->part, n_chunks, np, P, out, scale);
 }
 
 int backward(lct_cnn *n, const float *x, Shape s)
@@ -776,9 +865,9 @@ int backward(lct_cnn *n, const float *x, Shape s)
     if (tiles > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "cnn: too many voxel tiles");
     const int tc = int(std::min<long>(kMaxChunks, tiles));
     // layer 6
-    hipLaunchKernelGGL(wgrad_io_kernel<true>, dim3(vc, kTaps), dim3(256), 0, st, n->act[5], n->dy,
-                       n->part, s, vc);
-    reduce_into(n, vc, kTaps * kC, n->g + kW6);
+    hipLaunchKernelGGL(wgrad_io_kernel<true>, dim3(tc), dim3(256), 0, st, n->act[5], n->dy,
+                       n->part, s, int(tiles), tc);
+    reduce_into(n, tc, kTaps * kC, n->g + kW6);
     hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, nullptr, n->dy, n->part, nv, vc);
     reduce_into(n, vc, 1, n->g + kB6);
     hipLaunchKernelGGL(dgrad_out1_kernel, dim3(blocks_for(nv)), dim3(256), 0, st, n->dy,
@@ -799,9 +888,9 @@ int backward(lct_cnn *n, const float *x, Shape s)
         cur ^= 1;
     }
     // layer 1: grad[cur] = dL/d(pre-activation of layer 1)
-    hipLaunchKernelGGL(wgrad_io_kernel<false>, dim3(vc, kTaps), dim3(256), 0, st, n->grad[cur], x,
-                       n->part, s, vc);
-    reduce_into(n, vc, kTaps * kC, n->g + kW1);
+    hipLaunchKernelGGL(wgrad_io_kernel<false>, dim3(tc), dim3(256), 0, st, n->grad[cur], x,
+                       n->part, s, int(tiles), tc);
+    reduce_into(n, tc, kTaps * kC, n->g + kW1);
     hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, n->grad[cur], nullptr, n->part, nv,
                        vc);
     reduce_into(n, vc, kC, n->g + kB1);
@@ -933,8 +1022,7 @@ int lct_cnn_loss_grad(lct_cnn *n, const float *d_x, const float *d_t, int B, int
     const int vc = voxel_chunks(nv);
     hipLaunchKernelGGL(mse_kernel, dim3(vc), dim3(256), 0, n->ctx->stream, n->y, d_t, n->dy, nv, vc,
                        n->part);
-    hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(256), 0, n->ctx->stream, n->part, vc, 1L,
-                       n->loss, 1.0f / float(nv));
+    reduce_into(n, vc, 1, n->loss, 1.0f / float(nv));
     LCT_CHECK(backward(n, d_x, s));
     if (h_loss) {
         LCT_CHECK_HIP(hipMemcpyAsync(h_loss, n->loss, 4, hipMemcpyDeviceToHost, n->ctx->stream));
