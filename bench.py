@@ -5,12 +5,19 @@
 A step is one complete reconstruction (rebin -> filter -> backproject of all
 planned blocks) of the 512 x 512 x 256 clinical workload from 8192 views of a
 64 x 736 detector that are already resident in HBM.  The headline path is
-the mixed-precision one (f16 filtered projections, FP32 accumulation); bf16
-storage and the plain FP32 path (the reference's per-block loop on the GPU)
-are timed beside it, and the FP64 oracle is timed on the host on a bounded
-sample.  Optional: slab streaming from pinned host memory (--stream-slabs),
-the learned correction's inference (--no-learned to skip), the LISTCTRecons
-CLI end to end on the workload's files (--cli-e2e).
+the mixed-precision one (f16 filtered projections, FP32 accumulation); FP32
+storage on the same resident pipeline, bf16 storage and the plain FP32 path
+(the reference's per-block loop on the GPU) are timed beside it, and the FP64
+oracle is timed on the host on a bounded sample.  The large-detector workload
+(1024^2 x 512 from 12288 views of 96 x 896) is timed after it on the same
+path, in one piece and streamed in z-slabs (--large, --large-slab).
+Optional: slab streaming of the clinical workload from pinned host memory
+(--stream-slabs), the learned correction's inference (--no-learned to skip),
+the LISTCTRecons CLI end to end on the workload's files (--cli-e2e).
+
+The roofline of the backprojector is taken against VALU issue, the unit that
+binds it: the committed SQ_INSTS_VALU count of the launch (profiles/) over
+the launch time this run measures with HIP events.
 
 Prints one JSON line (rank 0).
 """
@@ -29,6 +36,11 @@ METRIC = "s/volume + backproj GUPS, 512²×256: FP32 vs mixed/learned at ≤1 HU
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 ENGINE_HZ = 2.4e9  # MI355X peak engine clock
 N_CU, N_SIMD = 256, 1024
+# VALU issue peak: a wave64 FP32 instruction occupies a SIMD for 2 cycles
+# (128 FP32 lanes per CU per clock), so 1024 SIMDs x 64 lanes / 2 cycles x
+# 2.4 GHz = 78.6e12 lane-operations/s (157.3 TFLOP/s if every one were an FMA)
+VALU_LANE_OPS_PEAK = N_SIMD * 64 / 2 * ENGINE_HZ
+KERNEL_SOURCE = os.path.join(ROOT, "listctrecons_amd", "csrc", "lct_resident.hip")
 
 
 def log(*a):
@@ -270,17 +282,102 @@ def load_profile(kernel_tag):
         return {}
 
 
-def issue_busy(prof, launch_s):
-    """VALU and LDS busy fractions of one launch from its SQ counters: wave64
-    FP32 VALU instructions take 2 cycles on a SIMD (1024 SIMDs), LDS-active
-    cycles are per CU (256 CUs), at the 2.4 GHz engine clock."""
-    if "sq_insts_valu_per_launch" not in prof:
-        return None
+def source_sha16(path=KERNEL_SOURCE):
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
+def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_tap, n_vox):
+    """Roofline of the backprojector launch against the unit that binds it,
+    VALU issue.  achieved = VALU lane-operations per launch (SQ_INSTS_VALU x 64
+    from the committed counters of this kernel and workload, which depend on
+    the code, not on timing) over the launch time measured in this run with
+    HIP events on the context stream; peak = 78.6e12 lane-ops/s.  So frac =
+    SQ_INSTS_VALU x 2 cycles / 1024 SIMDs / (launch s x 2.4 GHz).  Beside it:
+    LDS-active and bank-conflict fractions (SQ counters), measured HBM traffic
+    (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and its share of the 8 TB/s
+    peak, and the gather-byte rate of the north star's definition (2 rays x 4
+    taps x element size per update: the taps come from LDS, not HBM)."""
+    per_launch_updates = updates / launches
+    gather_bytes = (updates * 2 * 4 * bytes_per_tap + n_vox * 4) / launches
+    out = {"bound": "valu", "kernel": kernel}
+    insts = prof.get("sq_insts_valu_per_launch")
+    if insts:
+        achieved = insts * 64 / launch_s
+        out.update({
+            "achieved": round(achieved / 1e12, 2),
+            "peak": round(VALU_LANE_OPS_PEAK / 1e12, 2),
+            "unit": "T VALU lane-ops/s",
+            "frac": round(achieved / VALU_LANE_OPS_PEAK, 4),
+            "valu_insts_per_launch": insts,
+            "valu_lane_ops_per_update": round(insts * 64 / per_launch_updates, 2),
+        })
+    else:
+        out.update({"achieved": None, "peak": round(VALU_LANE_OPS_PEAK / 1e12, 2),
+                    "unit": "T VALU lane-ops/s", "frac": None})
+    traffic = prof.get("traffic_bytes_per_launch")
     cycles = launch_s * ENGINE_HZ
-    return {"valu_busy": round(prof["sq_insts_valu_per_launch"] * 2 / N_SIMD / cycles, 3),
-            "lds_busy": round(prof["sq_lds_idx_active_per_launch"] / N_CU / cycles, 3),
-            "lds_bank_conflict": round(prof["sq_lds_bank_conflict_per_launch"] / N_CU / cycles, 3),
-            "source": "profiles/" + prof.get("sq_source", "")}
+    out.update({
+        "traffic": traffic,
+        "traffic_frac_of_hbm_peak": round(traffic / launch_s / HBM_PEAK, 4) if traffic else None,
+        "lds_busy": (round(prof["sq_lds_idx_active_per_launch"] / N_CU / cycles, 3)
+                     if "sq_lds_idx_active_per_launch" in prof else None),
+        "lds_bank_conflict": (round(prof["sq_lds_bank_conflict_per_launch"] / N_CU / cycles, 3)
+                              if "sq_lds_bank_conflict_per_launch" in prof else None),
+        "gather_GBps": round(gather_bytes / launch_s / 1e9, 1),
+        "gather_definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update + "
+                             f"4 B per voxel written; {per_launch_updates:.4g} updates per launch; "
+                             f"served from LDS",
+        "launch_ms": round(launch_s * 1e3, 3),
+        "counters": "profiles/" + prof.get("sq_source", "") if prof else None,
+        "counters_match_source": (prof.get("source_sha16") == source_sha16()) if prof else None,
+    })
+    return out
+
+
+def large_workload(ctx, args, recon, scan, torch):
+    """The large-detector workload (by default 1024^2 x 512 from 12288 views of
+    96 x 896) on the headline path: 1 warm-up + 2 volumes back to back from
+    HBM-resident projections, the stage breakdown and the backprojector's
+    roofline, and z-slab streaming from pinned host memory."""
+    wl = scan.workload(args.large)
+    log(f"large workload {wl.name}: synthesising {wl.readings} views of {wl.nt}x{wl.ns}")
+    fan, a, c = recon.synth_scan(ctx, wl)
+    pl = recon.plan_helical(wl.prm(), a, c)
+    vol = torch.empty((pl.n_slices_recon, pl.ig.ny, pl.ig.nx), dtype=torch.float32, device="cuda")
+    opts = recon.preset(args.path, window=wl.window)
+    r = time_path(ctx, pl, opts, a, c, fan, vol, 2, 1)
+    log(f"  {r['s_per_volume']*1e3:.1f} ms/volume  stages {r['stage_ms']}")
+    storage = "f32" if args.path.startswith("fast_f32") else ("bf16" if "bf16" in args.path else "f16")
+    prof = load_profile(f"bp_fold_{storage}_{wl.name}")
+    n_vox = pl.n_slices_recon * pl.ig.nx * pl.ig.ny
+    res = {"workload": wl.name, "views": wl.readings, "detector": f"{wl.nt}x{wl.ns}",
+           "volume": f"{pl.ig.nx}x{pl.ig.ny}x{pl.n_slices_recon}", "path": args.path,
+           "steps": 2, "warmup": 1,
+           "s_per_volume": round(r["s_per_volume"], 6),
+           "stage_ms": {k: round(v, 3) for k, v in r["stage_ms"].items()},
+           "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"] * 1e-3) / 1e9, 2),
+           "roofline": backprojector_roofline(prof, prof.get("kernel", "bp_fold_kernel"),
+                                              r["updates"], r["bp_launches_per_volume"],
+                                              r["bp_ms_per_launch"] * 1e-3,
+                                              4 if storage == "f32" else 2, n_vox)}
+    if args.large_slab > 0:
+        host_fan = torch.empty(fan.shape, dtype=fan.dtype, pin_memory=True)
+        host_fan.copy_(fan)
+        svol = torch.empty_like(vol)
+        log(f"  streaming in slabs of {args.large_slab} blocks")
+        st_s, n_slabs, max_views, n_up = stream_slabs(ctx, pl, opts, a, c, host_fan, svol,
+                                                      args.large_slab, reps=1)
+        res["slab_streaming"] = {"blocks_per_slab": args.large_slab, "slabs": n_slabs,
+                                 "s_per_volume": round(st_s, 6), "max_views_per_slab": max_views,
+                                 "views_uploaded": n_up,
+                                 "identical_to_one_shot": bool(torch.equal(svol, vol))}
+        log(f"  {st_s*1e3:.1f} ms/volume streamed")
+        del host_fan, svol
+    del fan, vol
+    torch.cuda.empty_cache()
+    return res
 
 
 def hu_stats(x, ref, np, scan):
@@ -300,7 +397,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip timing the learned correction's inference")
-    ap.add_argument("--variants", default="mixed_bf16",
+    ap.add_argument("--variants", default="fast_f32,mixed_bf16",
                     help="comma-separated presets to time besides the headline path "
                          "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
     ap.add_argument("--stream-slabs", type=int, default=2,
@@ -311,6 +408,10 @@ def main():
                          "process start and HIP initialisation included)")
     ap.add_argument("--copy-probe-gib", type=float, default=0.0,
                     help="run one device copy of this many GiB first (PMC scale reference)")
+    ap.add_argument("--large", default="helical_96x896_1024x512",
+                    help="also time this larger workload (1 warm-up + 2 volumes, and slab "
+                         "streaming of --large-slab blocks); empty: off")
+    ap.add_argument("--large-slab", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU: the helical FDK path would shard by "
@@ -448,13 +549,10 @@ def main():
     updates = head["updates"]
     bp_s = head["bp_ms_per_launch"] * 1e-3
     launches = head["bp_launches_per_volume"]
-    alg_bytes = (updates * 2 * 4 * bytes_per_tap + pl.n_slices_recon * pl.ig.nx * pl.ig.ny * 4) \
-        / launches
-    achieved = alg_bytes / bp_s
     gups = updates / (head["stage_ms"]["backproject_ms"] * 1e-3) / 1e9
-    tag = f"bp_fold_{storage}_{w.name}"
-    prof = load_profile(tag)
-    traffic = prof.get("traffic_bytes_per_launch")
+    prof = load_profile(f"bp_fold_{storage}_{w.name}")
+    roof = backprojector_roofline(prof, prof.get("kernel", "bp_fold_kernel"), updates, launches,
+                                  bp_s, bytes_per_tap, pl.n_slices_recon * pl.ig.nx * pl.ig.ny)
 
     out = {
         "metric": METRIC,
@@ -475,23 +573,7 @@ def main():
                    "volume": f"{pl.ig.nx}x{pl.ig.ny}x{pl.n_slices_recon}", "path": args.path},
         "backproject_gups": round(gups, 2),
         "stage_ms": {k: round(v, 3) for k, v in head["stage_ms"].items()},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "bp_fold_kernel",
-            "achieved": round(achieved / 1e9, 1),
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": traffic,
-            # measured HBM bytes (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)
-            # over this run's launch time: the share of HBM bandwidth really used
-            "traffic_frac_of_peak": (round(traffic / bp_s / HBM_PEAK, 4) if traffic else None),
-            "definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update "
-                          f"+ 4 B per voxel written, per launch; {updates / launches:.4g} updates "
-                          f"per launch; the taps are read from LDS, so frac > 1 is possible",
-            # what binds the kernel instead: VALU and LDS issue
-            "issue": issue_busy(prof, bp_s),
-        },
+        "roofline": roof,
         "pcie": {"h2d_ms": round(h2d_s * 1e3, 3),
                  "h2d_GBps": round(fan.numel() * 4 / h2d_s / 1e9, 1),
                  "d2h_ms": round(d2h_s * 1e3, 3),
@@ -512,6 +594,14 @@ def main():
         out["fp32_plain_stage_ms"] = {k: round(v, 3) for k, v in plain["stage_ms"].items()}
         out["speedup_vs_fp32_plain"] = round(plain["s_per_volume"] / head["s_per_volume"], 3)
         out["hu_vs_fp32_plain"] = hu_stats(head_vol, plain_vol, np, scan)
+    if "fast_f32" in variants and args.path != "fast_f32":
+        # the same resident pipeline and backprojector with FP32 storage and the
+        # FP32 direct filter: what the reduced precision alone buys
+        out["speedup_vs_fp32_resident"] = round(variants["fast_f32"]["s_per_volume"]
+                                                / head["s_per_volume"], 3)
+
+    if args.large and args.large != w.name:
+        out["large_workload"] = large_workload(ctx, args, recon, scan, torch)
 
     if not args.no_cpu_baseline:
         log("timing the FP64 oracle on the host (bounded sample)")
