@@ -180,6 +180,18 @@ int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int ns, int nt,
 int lct_backproject(lct_ctx *ctx, int kind, const lct_ct_geom *g, const lct_img_geom *ig,
                     const float *d_q, const float *h_betas, float *d_img);
 
+/* lct_backproject without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops).  Same arguments, kernels and results
+ * as lct_backproject; its tables live in a fixed device array of the library
+ * (128.4 KiB per GPU, held until the process ends) and reach it through
+ * kernel arguments.  Limit: n_angle + n_proj_turn/2 + 1 <= 16384, else
+ * LCT_ERR_UNSUPPORTED.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (other streams or contexts) race; calls in one stream
+ * are ordered. */
+int lct_backproject_noalloc(lct_ctx *ctx, int kind, const lct_ct_geom *g, const lct_img_geom *ig,
+                            const float *d_q, const float *h_betas, float *d_img);
+
 /* ------------------------------------------------ whole-scan pipeline */
 typedef struct {
     int pipeline;    /* LCT_PIPE_BLOCKS: the reference's per-block loop

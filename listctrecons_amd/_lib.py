@@ -128,6 +128,7 @@ SIGNATURES = {
     "lct_weighting": (C.c_int, [_vp, C.c_int, _P(CTGeom), _vp]),
     "lct_filter": (C.c_int, [_vp, _P(CTGeom), _fp, _vp, C.c_long]),
     "lct_backproject": (C.c_int, [_vp, C.c_int, _P(CTGeom), _P(ImgGeom), _vp, _fp, _vp]),
+    "lct_backproject_noalloc": (C.c_int, [_vp, C.c_int, _P(CTGeom), _P(ImgGeom), _vp, _fp, _vp]),
     "lct_filter_to_bp_layout": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _vp,
                                           C.c_long, _vp]),
     "lct_reconstruct": (C.c_int, [_vp, _P(Plan), _P(ReconOpts), _fp, _fp, _vp, C.c_long, _vp]),

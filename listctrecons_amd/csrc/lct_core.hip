@@ -170,6 +170,7 @@ int lct_ctx_create(int device, lct_ctx **out)
         return set_error(LCT_ERR_INVALID, "lct_ctx_create: device " + std::to_string(device) +
                                               " not present (" + std::to_string(n) + " devices)");
     LCT_CHECK_HIP(hipSetDevice(device));
+    LCT_CHECK(noalloc_scratch_init(device));
     lct_ctx *c = new lct_ctx();
     c->device = device;
     c->stream = nullptr;

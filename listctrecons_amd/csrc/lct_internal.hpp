@@ -77,6 +77,7 @@ void clear_error();
 
 // Returns a device buffer of at least `bytes` in slot `slot` (grown, never shrunk).
 int scratch(lct_ctx *ctx, int slot, size_t bytes, void **out);
+int noalloc_scratch_init(int device);
 
 // Host-to-device copy on the context stream, ordered after earlier work on it;
 // returns once `h` has been read (into the pinned staging ring), without

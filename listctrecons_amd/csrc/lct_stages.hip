@@ -10,6 +10,8 @@
 // coordinate is split into an integer texel index and an FP32 weight that
 // the host evaluates in double, and texels outside the array read as 0.
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "lct_internal.hpp"
@@ -534,6 +536,93 @@ int backproject_block(lct_ctx *ctx, int kind, const lct_ct_geom &g, const lct_im
     return LCT_OK;
 }
 
+// ------------------------------------------- allocation-free backprojector
+// lct_backproject_noalloc keeps its tables (cos and sin of the block's tube
+// angles, then the 32 slices' view windows) in a file-scope device array of
+// fixed size (bp_noalloc_scratch, 128.4 KiB): one copy per GPU, held until
+// the process ends, shared by every call on that GPU.  The array's name is
+// only a host-side stand-in; its address on each GPU comes from
+// hipGetSymbolAddress, taken once per device when a context is created there
+// (noalloc_scratch_init), and host code passes only that address to kernels.  The tables reach the array through
+// kernel arguments (no staging buffer, no copy from host memory).
+constexpr int kNoAllocMaxAngles = 16384;  // n_angle + n_proj_turn / 2 + 1 at most
+constexpr size_t kNoAllocBytes = size_t(2) * kNoAllocMaxAngles * 4 + 3 * 32 * 4;
+static __device__ __align__(256) unsigned char bp_noalloc_scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Up to kArgWords 32-bit words of table through a kernel argument, split
+// between two destinations: words [0, n0) -> dst0, [n0, n0 + n1) -> dst1.
+constexpr int kArgWords = 960;
+struct WordChunk {
+    unsigned v[kArgWords];
+};
+
+__global__ __launch_bounds__(256) void fill_words_kernel(WordChunk c, int n0, unsigned *dst0, int n1,
+                                                         unsigned *dst1)
+{
+    const int i = int(blockIdx.x) * 256 + int(threadIdx.x);
+    if (i < n0)
+        dst0[i] = c.v[i];
+    else if (i < n0 + n1)
+        dst1[i - n0] = c.v[i];
+}
+
+int backproject_block_noalloc(lct_ctx *ctx, int kind, const lct_ct_geom &g, const lct_img_geom &ig,
+                              const float *d_q, const float *h_betas, float *d_img)
+{
+    if (ig.nz != LCT_BLOCK_SLICES)
+        return set_error(LCT_ERR_INVALID, "backproject: ImgGeom nz must be 32 (BLOCK_SLICES)");
+    const int nb = g.n_angle + g.n_proj_turn / 2 + 1;
+    if (nb > kNoAllocMaxAngles)
+        return set_error(LCT_ERR_UNSUPPORTED,
+                         "lct_backproject_noalloc: n_angle + n_proj_turn/2 + 1 = " + std::to_string(nb) +
+                             " exceeds its fixed scratch of " + std::to_string(kNoAllocMaxAngles) +
+                             " angles (use lct_backproject)");
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch_ptr[ctx->device])
+        return set_error(LCT_ERR_INVALID, "lct_backproject_noalloc: scratch of this device not set up");
+    float *d_cos = static_cast<float *>(g_scratch_ptr[ctx->device]);
+    float *d_sin = d_cos + nb;
+    int *d_win = reinterpret_cast<int *>(d_sin + nb);
+    WordChunk c;
+    // cos / sin of each float32 tube angle, evaluated in double (as angle_tables)
+    constexpr int kHalf = kArgWords / 2;
+    for (int i0 = 0; i0 < nb; i0 += kHalf) {
+        const int n = std::min(kHalf, nb - i0);
+        for (int i = 0; i < n; i++) {
+            c.v[i] = __builtin_bit_cast(unsigned, float(std::cos(double(h_betas[i0 + i]))));
+            c.v[n + i] = __builtin_bit_cast(unsigned, float(std::sin(double(h_betas[i0 + i]))));
+        }
+        hipLaunchKernelGGL(fill_words_kernel, dim3(unsigned((2 * n + 255) / 256)), dim3(256), 0,
+                           ctx->stream, c, n, reinterpret_cast<unsigned *>(d_cos + i0), n,
+                           reinterpret_cast<unsigned *>(d_sin + i0));
+        LCT_CHECK_HIP(hipGetLastError());
+    }
+    // kernel layout of the windows: (a_min, a_max) of the 32 slices, then a_middle[32]
+    int amin[32], amax[32], amid[32];
+    bp_windows(g, ig, amin, amax, amid);
+    for (int iz = 0; iz < 32; iz++) {
+        c.v[2 * iz] = unsigned(amin[iz]);
+        c.v[2 * iz + 1] = unsigned(amax[iz]);
+        c.v[64 + iz] = unsigned(amid[iz]);
+    }
+    hipLaunchKernelGGL(fill_words_kernel, dim3(1), dim3(256), 0, ctx->stream, c, 96,
+                       reinterpret_cast<unsigned *>(d_win), 0, static_cast<unsigned *>(nullptr));
+    LCT_CHECK_HIP(hipGetLastError());
+    const BpGeom G = make_bp_geom(g, ig);
+    const dim3 grid(unsigned((ig.nx + 255) / 256), unsigned(ig.ny), unsigned(ig.nz));
+    if (kind == LCT_BP_PI_ORI)
+        hipLaunchKernelGGL(bp_pi_ori_literal_kernel, grid, dim3(256), 0, ctx->stream, d_q, d_cos,
+                           d_sin, d_win, d_img, G);
+    else
+        hipLaunchKernelGGL(bp_helical_literal_kernel, grid, dim3(256), 0, ctx->stream, d_q, d_cos,
+                           d_sin, d_win, d_img, G);
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
 bool geom_ok(const lct_ct_geom *g)
 {
     return g && g->ns > 0 && g->nt > 0 && g->n_angle > 0 && g->n_angle_add >= g->n_angle &&
@@ -544,6 +633,21 @@ bool geom_ok(const lct_ct_geom *g)
 }  // namespace
 
 namespace lct {
+// Device address of the allocation-free backprojector's scratch array on
+// `device` (the current device), taken once per device.
+int noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return set_error(LCT_ERR_UNSUPPORTED, "device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return LCT_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(bp_noalloc_scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(bp_noalloc_scratch)");
+    g_scratch_ptr[device] = p;
+    return LCT_OK;
+}
+
 int backproject_block_public(lct_ctx *ctx, int kind, const lct_ct_geom &g,
                              const lct_img_geom &ig, const float *d_q, const float *h_betas,
                              float *d_img)
@@ -601,6 +705,16 @@ int lct_backproject(lct_ctx *ctx, int kind, const lct_ct_geom *g, const lct_img_
         return set_error(LCT_ERR_INVALID, "lct_backproject: bad argument");
     LCT_CHECK_HIP(hipSetDevice(ctx->device));
     return backproject_block(ctx, kind, *g, *ig, d_q, h_betas, d_img);
+}
+
+int lct_backproject_noalloc(lct_ctx *ctx, int kind, const lct_ct_geom *g, const lct_img_geom *ig,
+                            const float *d_q, const float *h_betas, float *d_img)
+{
+    if (!ctx || !geom_ok(g) || !ig || !d_q || !h_betas || !d_img || kind < 0 || kind > 1 ||
+        ig->nx <= 0 || ig->ny <= 0)
+        return set_error(LCT_ERR_INVALID, "lct_backproject_noalloc: bad argument");
+    LCT_CHECK_HIP(hipSetDevice(ctx->device));
+    return backproject_block_noalloc(ctx, kind, *g, *ig, d_q, h_betas, d_img);
 }
 
 }  // extern "C"

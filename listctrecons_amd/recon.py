@@ -236,6 +236,18 @@ class Context:
         check(lib().lct_backproject(self.h, kind, C.byref(g), C.byref(ig), _dev(q), _fptr(b),
                                     _dev(img)))
 
+    def backproject_noalloc(self, kind: int, g: _lib.CTGeom, ig: _lib.ImgGeom, q, betas,
+                            img) -> None:
+        """lct_backproject_noalloc: backproject() without any allocation (its
+        tables in the library's fixed per-GPU array)."""
+        b = np.ascontiguousarray(betas, np.float32)
+        if b.size < g.n_angle + g.n_proj_turn // 2 + 1:
+            raise ValueError("betas needs n_angle + n_proj_turn/2 + 1 values")
+        _check_buf(q, g.n_angle * g.nt * g.ns, "q")
+        _check_buf(img, ig.nz * ig.ny * ig.nx, "img")
+        check(lib().lct_backproject_noalloc(self.h, kind, C.byref(g), C.byref(ig), _dev(q),
+                                            _fptr(b), _dev(img)))
+
     # -------------------------------------------------------- pipeline
     def reconstruct(self, plan: _lib.Plan, opts: Options, angles: np.ndarray,
                     couch: np.ndarray, fan, vol=None):
