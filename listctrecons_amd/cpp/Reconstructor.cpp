@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -68,17 +69,39 @@ namespace {
     std::exit(1);
 }
 
-// Identity of a streamed reconstruction: a checkpoint is only resumed by a
-// run with the same plan shape, slab size and options.
-std::string checkpoint_key(const lct_plan &p, const lct_recon_opts &o, int per)
+// 64-bit FNV-1a over bytes, continuing from h.
+uint64_t fnv1a(uint64_t h, const void *data, size_t n)
 {
-    char buf[256];
+    const unsigned char *b = static_cast<const unsigned char *>(data);
+    for (size_t i = 0; i < n; i++) {
+        h ^= b[i];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+// Identity of a streamed reconstruction: a checkpoint is only resumed by a
+// run with the same plan (every geometry field; lct_plan holds only 4-byte
+// members, so its bytes are its fields), slab size and options, the same
+// projection file (path and size) and the same tube-angle and couch tables.
+std::string checkpoint_key(const lct_plan &p, const lct_recon_opts &o, int per,
+                           const std::string &data_path, long long data_bytes,
+                           const float *angles, const float *couch)
+{
+    static_assert(sizeof(lct_plan) % 4 == 0, "lct_plan holds 4-byte fields");
+    const uint64_t h0 = 1469598103934665603ULL;
+    const uint64_t geom = fnv1a(h0, &p, sizeof p);
+    uint64_t tables = fnv1a(h0, angles, size_t(p.n_readings) * sizeof(float));
+    tables = fnv1a(tables, couch, size_t(p.n_readings) * sizeof(float));
+    char buf[320];
     std::snprintf(buf, sizeof buf,
-                  "plan %d %d %d %d %d slab %d opts %d %d %d %d %d %d %d %d", p.n_blocks,
-                  p.n_slices_recon, p.ig.nx, p.ig.ny, p.ctg.n_angle_add, per, o.pipeline,
-                  o.rebin_kind, o.weight_kind, o.window, o.filter_algo, o.storage, o.bp_kind,
-                  o.interp);
-    return buf;
+                  "plan %d %d %d %d %d slab %d opts %d %d %d %d %d %d %d %d geom %016llx "
+                  "tables %016llx data %lld ",
+                  p.n_blocks, p.n_slices_recon, p.ig.nx, p.ig.ny, p.ctg.n_angle_add, per,
+                  o.pipeline, o.rebin_kind, o.weight_kind, o.window, o.filter_algo, o.storage,
+                  o.bp_kind, o.interp, (unsigned long long)geom, (unsigned long long)tables,
+                  data_bytes);
+    return buf + data_path;
 }
 
 // Blocks already done by an earlier run (0 when there is no checkpoint);
@@ -126,7 +149,11 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
     // resume point
     const std::string out_path = mr->fi_()->output_dir() + mr->fi_()->output_name();
     const std::string ck_path = out_path + ".ckpt";
-    const std::string key = checkpoint_key(plan, opts, per);
+    const std::string in_path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
+    struct stat sb;
+    if (stat(in_path.c_str(), &sb) != 0) fail("raw data file " + in_path + " not found");
+    const std::string key =
+        checkpoint_key(plan, opts, per, in_path, (long long)sb.st_size, angles, couch);
     int start = 0;
     FILE *out = nullptr;
     if (checkpoint) {
@@ -145,7 +172,6 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
 
     // device buffers sized for the largest slab; the projection file must hold
     // every view the remaining slabs read (checked before any device work)
-    const std::string in_path = mr->fi_()->in_file_dir() + mr->fi_()->data_file_name();
     long max_views = 0, last_view = 0;
     for (int b0 = start; b0 < plan.n_blocks; b0 += per) {
         long v0 = 0, v1 = 0;
@@ -155,8 +181,6 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
         max_views = std::max(max_views, v1 - v0);
         last_view = std::max(last_view, v1);
     }
-    struct stat sb;
-    if (stat(in_path.c_str(), &sb) != 0) fail("raw data file " + in_path + " not found");
     if (off_t(sb.st_size) < off_t(last_view) * off_t(view) * off_t(sizeof(float)))
         fail("projection file " + in_path + " holds " +
              std::to_string(sb.st_size / (off_t(view) * off_t(sizeof(float)))) +

@@ -50,6 +50,14 @@ struct lct_ctx {
     // (scratch slots A / B, untouched until the next reconstruct call):
     // layout 1: rows [view][nt][ns] f32; layout 2: Q [view][cp][rp] of
     // `storage`; 0: nothing.  The block's views start at view0.
+    // What the resident Q buffer was last cleared for: its guard slots are
+    // zero while buffer, layout and extent are unchanged.
+    struct QZero {
+        const void *ptr = nullptr;
+        int cp = 0, rp = 0;
+        size_t esz = 0, bytes = 0;
+    } qzero;
+
     struct LastBlock {
         int layout = 0, storage = 0, n_angle = 0, nt = 0, ns = 0, cp = 0, rp = 0;
         long view0 = 0;

@@ -664,7 +664,16 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     const size_t esz = o.storage == LCT_F32 ? 4 : 2;
     const size_t q_view = size_t(cp) * rp * esz;  // bytes of one view in Q
     LCT_CHECK(scratch(ctx, kScratchB, q_view * R.nviews, &d_q));
-    LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_view * R.nviews, ctx->stream));
+    // The filters write every channel and row slot of the views they fill and
+    // never the guard slots, so Q's guards stay zero from one call to the
+    // next: Q is cleared only when its buffer, its layout or its extent
+    // changed since it was last cleared.
+    lct_ctx::QZero &qz = ctx->qzero;
+    if (qz.ptr != d_q || qz.cp != cp || qz.rp != rp || qz.esz != esz ||
+        qz.bytes < q_view * R.nviews) {
+        LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_view * R.nviews, ctx->stream));
+        qz = lct_ctx::QZero{d_q, cp, rp, esz, q_view * R.nviews};
+    }
     // rebin and filter of the resident views [v0, v1) on the context stream
     auto prep = [&](long v0, long v1) -> int {
         LCT_CHECK(tic());

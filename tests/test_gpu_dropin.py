@@ -135,6 +135,34 @@ def test_cli_slab_streaming_and_resume(three_blocks):
     assert np.array_equal(image("resumed"), whole)
 
 
+def test_cli_checkpoint_of_another_scan_is_refused(three_blocks, tmp_path):
+    """A checkpoint records the plan's geometry, the tube-angle and couch
+    tables and the projection file (path and size): a stopped run's checkpoint
+    is not resumed after the angle file changed (same sizes), and is resumed
+    once it is restored."""
+    w, dirs, ref = three_blocks
+    fan = np.fromfile(os.path.join(dirs["resumed"][0], "proj.raw"), np.float32)
+    d = str(tmp_path)
+    prm = scan.write_scan_files(w, d, fan.reshape(-1, w.nt, w.ns))
+    run = lambda *extra: subprocess.run([CLI, prm, "Helicalfdkramprebingpu-resident-ckpt", *extra],
+                                        stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                        text=True, timeout=300)
+    r = run("--stop-after-slabs", "1")
+    assert r.returncode == 3, r.stdout[-2000:]
+    ang_path = os.path.join(d, "angles.raw")
+    ang = np.fromfile(ang_path, np.float32)
+    (ang + np.float32(1e-3)).tofile(ang_path)
+    r = run()
+    assert r.returncode == 1, r.stdout[-2000:]
+    assert "belongs to another reconstruction" in r.stdout
+    ang.tofile(ang_path)
+    r = run()
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "Resuming after block 0" in r.stdout
+    img = np.fromfile(os.path.join(d, "image.raw"), np.float32).reshape(ref.shape)
+    assert rel_rmse(img, ref) <= 1e-4
+
+
 def test_gpu_memory_ctrl_semantics():
     """GPUMemoryCtrl<T> of the drop-in headers on the device (program built
     from tests/native/gpu_memory_ctrl.cpp by the package build)."""
