@@ -102,3 +102,39 @@ def test_pi_ori_weight():
     expect = np.sqrt(a)[None, :] / np.sqrt(a[None, :] + (t[:, None] - gam[None, :] * g.pitch
                                                         / (2 * math.pi)) ** 2)
     np.testing.assert_allclose(w[3], expect, rtol=1e-14)
+
+
+def test_pi_ori_weight_is_the_cone_cosine_of_the_rebinned_ray():
+    """Pins lco_weight_pi_ori by the geometry it encodes rather than by its
+    formula.  A parallel-rebinned ray at channel offset s comes from a source
+    at horizontal distance D = sqrt(dso^2 - s^2) from the virtual detector
+    through the axis, displaced along z by gamma P / (2 pi) with
+    gamma = asin(s / dso) (the helix advances P per turn, the fan angle gamma
+    is gamma / (2 pi) of a turn).  The weight is the cosine of the ray's cone
+    angle towards detector row height t: w = D / sqrt(D^2 + (t - z_src)^2).
+
+    From the weights alone, per channel: 1/w^2 - 1 = (t - z_src)^2 / D^2 is a
+    quadratic in the row height t, so a fit through the rows recovers the
+    source's z (vertex) and D (curvature), which must match the geometry; at
+    the central channel the recovered row heights are spaced P / (2 nt)
+    (CUDA/PreWeighting.cu:8-29 scales the rows by 2 nt; the PI-original
+    rebin, Rebiner.cu:156-179, by 2 (nt - 1)).  A flipped sign of the gamma
+    term, a linear gamma = s / dso, a dropped s^2 in D, a 2 pi for 2 pi / P
+    or rows scaled by 2 (nt - 1) each fail one of these checks."""
+    g = geom(ns=41, nt=12, ws=20.0, dso=60.0, ds=1.2, pitch=24.0)
+    w = O.weight_pi_ori(g, np.ones((g.n_angle, g.nt, g.ns)))[0]  # [nt][ns]
+    row_step = g.pitch / (2 * g.nt)
+    # central channel (s = 0, gamma = 0): 1/w^2 - 1 = t^2 / dso^2
+    t_abs = g.dso * np.sqrt(1.0 / w[:, 20] ** 2 - 1.0)
+    t_signed = np.where(np.arange(g.nt) < g.wt, -t_abs, t_abs)
+    np.testing.assert_allclose(np.diff(t_signed), row_step, rtol=1e-9)
+    assert abs(row_step - g.pitch / (2 * (g.nt - 1))) > 1e-3  # the two scalings differ here
+    t = (np.arange(g.nt) - g.wt) * row_step
+    for is_ in (0, 5, 13, 27, 40):
+        s = (is_ - g.ws) * g.ds
+        a, b, c = np.polyfit(t, 1.0 / w[:, is_] ** 2 - 1.0, 2)
+        z_src, D = -b / (2 * a), 1.0 / math.sqrt(a)
+        assert D == pytest.approx(math.sqrt(g.dso ** 2 - s ** 2), rel=1e-9)
+        assert z_src == pytest.approx(math.asin(s / g.dso) * g.pitch / (2 * math.pi),
+                                      rel=1e-8, abs=1e-10)
+        assert c == pytest.approx(z_src ** 2 / D ** 2, rel=1e-7, abs=1e-12)
