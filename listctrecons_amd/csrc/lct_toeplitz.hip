@@ -4,13 +4,17 @@
 //   Q[(v,r)][ch] = sum_k T[ch][k] P[(v,r)][k],   T[ch][k] = h[ch - k + ns]
 //
 // C (M = channels) = A (M x K, the Toeplitz taps) * B (K x N, N = every row
-// of every view).  The taps are split three ways, h = h1 + h2 + h3 (bf16
-// each, exact to ~2^-25), the data two ways, x = x1 + x2 (~2^-17), and
-//   h1 x1 + h1 x2 + h2 x1 + h3 x1
-// is accumulated in FP32 with four v_mfma_f32_16x16x32_bf16 per fragment.
-// The tap split matters: its error is the same for every row, so it would
-// bias the filter's DC response; the data split's error is random per sample
-// and averages out in the backprojection.  The result is written in the
+// of every view).  The taps are split two ways, h = h1 + h2 (bf16 each), the
+// data two ways, x = x1 + x2 (~2^-17), and
+//   h1 x1 + h1 x2 + h2 x1
+// is accumulated in FP32 with three v_mfma_f32_16x16x32_bf16 per fragment.
+// The tap split's error is the same for every row, so it would bias the
+// filter's DC response: h2 is rounded with error diffusion along the tap
+// index (each tap's residual carried into the next one's rounding), so the
+// residuals' sum over any contiguous run of taps -- every output channel's
+// DC gain is such a sum -- stays within one h2 rounding step instead of
+// growing with the run.  The data split's error is random per sample and
+// averages out in the backprojection.  The result is written in the
 // backprojector's channel-major layout Q[view][slot_ch][slot_row] (see
 // lct_resident.hip) as f32, f16 or bf16.
 //
@@ -40,7 +44,7 @@ constexpr int kBRow = 40;    // LDS row of one B column: 32 k + 8 pad (bf16)
 constexpr int kRowPadT = 4;  // zero row slots before row 0 in Q (as lct_resident.hip)
 
 struct ToeplitzArgs {
-    const unsigned short *frag;  // [3][n_diag][64 lanes][8] bf16 bits
+    const unsigned short *frag;  // [2][n_diag][64 lanes][8] bf16 bits (h1, h2)
     long frag_stride;            // n_diag * 64 * 8
     const float *in;             // P [(v,r)][ns] f32
     void *q;                     // Q [v][cp][rp]
@@ -141,7 +145,7 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
             if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, xb);
             // A fragments of this wave's two 16-channel slices: diagonal index
             // (m - k0 + kp - 32) / 16 of the precomputed table
-            bf16x8 a1[2], a2[2], a3[2];
+            bf16x8 a1[2], a2[2];
 #pragma unroll
             for (int a = 0; a < 2; a++) {
                 const int m = m0 + (2 * wave + a) * 16;
@@ -149,7 +153,6 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
                 const unsigned short *f = A.frag + (di * 64 + lane) * 8;
                 a1[a] = *reinterpret_cast<const bf16x8 *>(f);
                 a2[a] = *reinterpret_cast<const bf16x8 *>(f + A.frag_stride);
-                a3[a] = *reinterpret_cast<const bf16x8 *>(f + 2 * A.frag_stride);
             }
 #pragma unroll
             for (int f = 0; f < 4; f++) {
@@ -158,7 +161,6 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
                 const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(bl + (buf * kTN + c) * kBRow + lk);
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
-                    acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[a], b_hi, acc[a][f], 0, 0, 0);
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[a], b_hi, acc[a][f], 0, 0, 0);
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_lo, acc[a][f], 0, 0, 0);
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_hi, acc[a][f], 0, 0, 0);
@@ -213,7 +215,18 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
     // diagonals d = m - k0 from -(kp - 32) to mp - 16 in steps of 16
     const int n_diag = (mp + kp - kTK) / 16;
     const long stride = long(n_diag) * 64 * 8;
-    std::vector<unsigned short> frag(3 * size_t(stride));
+    // h = h1 + h2 per tap index, h2 rounded with the residual of the previous
+    // taps carried along (error diffusion)
+    std::vector<unsigned short> t1(2 * size_t(ns)), t2(2 * size_t(ns));
+    double carry = 0.0;
+    for (int idx = 0; idx < 2 * ns; idx++) {
+        const double h = double(h_taps[idx]);
+        t1[idx] = host_bf16_rn(float(h));
+        const double want = (h - host_bf16_f(t1[idx])) + carry;
+        t2[idx] = host_bf16_rn(float(want));
+        carry = want - host_bf16_f(t2[idx]);
+    }
+    std::vector<unsigned short> frag(2 * size_t(stride));
     for (int di = 0; di < n_diag; di++) {
         const int d = di * 16 - (kp - kTK);
         for (int lane = 0; lane < 64; lane++)
@@ -221,16 +234,10 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
                 // T[m + i][k0 + kk] = h[d + i - kk + ns]
                 const int i = lane & 15, kk = (lane >> 4) * 8 + j;
                 const int idx = d + i - kk + ns;
-                const double h = (idx >= 0 && idx < 2 * ns) ? double(h_taps[idx]) : 0.0;
-                const unsigned short b1 = host_bf16_rn(float(h));
-                const double r1 = h - host_bf16_f(b1);
-                const unsigned short b2 = host_bf16_rn(float(r1));
-                const double r2 = r1 - host_bf16_f(b2);
-                const unsigned short b3 = host_bf16_rn(float(r2));
+                const bool in = idx >= 0 && idx < 2 * ns;
                 const size_t o = (size_t(di) * 64 + lane) * 8 + j;
-                frag[o] = b1;
-                frag[stride + o] = b2;
-                frag[2 * stride + o] = b3;
+                frag[o] = in ? t1[idx] : 0;
+                frag[stride + o] = in ? t2[idx] : 0;
             }
     }
     void *dt = nullptr;
