@@ -3,7 +3,8 @@
 // Every parallel view the planned blocks need is rebinned once and filtered
 // once into Q, an HBM-resident array in the backprojector's layout:
 //     Q[view][slot_ch][slot_row],  slot_ch = channel + 1  (cp = ns + 3 slots),
-//                                  slot_row = row + 4     (rp = nt + 8 slots, nt % 8 == 0)
+//                                  slot_row = row + 4     (rp = nt + 8 slots, nt % 8 == 0;
+//                                                          f32: rp = nt + 9, odd)
 // with zero guard slots, stored as f32, f16 or bf16.  A view is then one
 // contiguous block of cp*rp elements, and the channel window a tile of voxel
 // columns needs is one contiguous range.
@@ -146,9 +147,11 @@ struct FoldBpArgs {
     const void *q;
     size_t view_stride;  // cp * rp elements
     int rp, cp, W, H;    // channel window of W = 2H + 2 slots
-    int lp;              // LDS row pitch of a staged channel (rp + 1: odd, so the
-                         // channels that neighbouring lanes read fall in different banks)
+    int lp;              // LDS row pitch of a staged channel (rp + 1, or rp for D kernels:
+                         // odd, so the channels that neighbouring lanes read fall in
+                         // different banks)
     int asin_poly;       // every voxel column has |p_hat| / dso <= 0.7: asin_poly is valid
+    int units;           // D kernels: 16-byte units copied per view window (see bp_fold_kernel)
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
     int n_pairs;               // pairs j of a block: a_middle[0] - h <= j < a_middle[31]
     float *vol;
@@ -225,16 +228,27 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // cs: (cos, sin) of each Q view's tube angle; zmask: per pair j - jlo, bit z
 // set when a_middle[z] - h <= j < a_middle[z] (the pair lies in slice z's
 // window; the same for every block); win: per (block, tile, pair) the first
-// channel slots of the two views' LDS windows (s1 | s2 << 16).  All three are
-// read with wave-uniform indices, so they arrive in scalar registers.
-// f32 storage stages 4 chunks per thread and would use 84 VGPRs (2 workgroups
-// per CU); capped at 80 it keeps 3 (4 spilled, 37.7 -> 34.9 ms on the
-// clinical scan).  16-bit storage needs 71 and no cap.
-template <typename T, int NPRE, int LP, bool G = false>
+// channel slots of the two views' LDS windows (.x = s1 | s2 << 16) and the
+// slices on which every column of the tile has both rays on the detector
+// (.y, see bp_windows_kernel).  All three are read with wave-uniform
+// indices, so they arrive in scalar registers.
+// f32 storage staged through registers takes 4 chunks per thread and would
+// use 84 VGPRs (2 workgroups per CU); capped at 80 it keeps 3 (4 spilled).
+// 16-bit storage needs 71 and no cap.
+//
+// D (f32 storage only): the windows are copied from Q into LDS by LDS-DMA
+// loads (global_load_lds_dwordx4), with no registers or conversion in
+// between.  Q then carries the odd pitch itself (rp = lp = nt + 9), so a
+// view's window is one contiguous range, copied in 16-byte units from the
+// 16-byte boundary at or below its start: the window's first element lands
+// `shift` = (start mod 4) floats into its LDS region.  The two regions are
+// 4 Up floats apart (Up = units per view rounded up to 64), so each wave's
+// 64 units belong to one view and fill 1 KB of LDS in lane order.
+template <typename T, int NPRE, int LP, bool G = false, bool D = false>
 __global__ __launch_bounds__(kBpThreads)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
                                                       const unsigned *__restrict__ zmask,
-                                                      const unsigned *__restrict__ win)
+                                                      const uint2 *__restrict__ win)
 {
     extern __shared__ float lds[];
     constexpr int CE = Chunk<T>::kElems;
@@ -310,10 +324,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
             if (live[i]) chunk_to_lds_odd<T>(buf + (offs[i] >> 16), pre[i]);
     };
 
+    // D: LDS-DMA copy of pair j's two windows (views j and j + h)
+    const int Up = (A.units + 63) & ~63;
+    auto first_elem = [&](int v, int sl) -> size_t {
+        return size_t(qoff + v) * A.view_stride + size_t(sl) * rp;
+    };
+    auto dma = [&](int j, int s1, int s2) {
+        const float *Qf = static_cast<const float *>(A.q);
+        const size_t ea = first_elem(j, s1) & ~size_t(3), eb = first_elem(j + A.h, s2) & ~size_t(3);
+        const int lane = int(threadIdx.x & 63);
+        const int u_wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) & ~63);
+        for (int u0 = u_wave; u0 < 2 * Up; u0 += kBpThreads) {
+            const bool vb = u0 >= Up;  // the same on every lane of the wave
+            const int lu = u0 - (vb ? Up : 0) + lane;
+            if (lu < A.units)
+                __builtin_amdgcn_global_load_lds(
+                    (__attribute__((address_space(1))) void *)(Qf + (vb ? eb : ea) + 4 * size_t(lu)),
+                    (__attribute__((address_space(3))) void *)(lds + 4 * u0), 16, 0, 0);
+        }
+    };
+
     const float2 *csq = cs + qoff;
-    const unsigned *wt = win + size_t(blk * tiles + t) * A.n_pairs;  // index j - jlo
-    if constexpr (!G) {
-        const unsigned w = wt[0];
+    const uint2 *wt = win + size_t(blk * tiles + t) * A.n_pairs;  // index j - jlo
+    if constexpr (D) {
+        const unsigned w = wt[0].x;
+        dma(jlo, int(w & 0xffffu), int(w >> 16));
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    } else if constexpr (!G) {
+        const unsigned w = wt[0].x;
         fetch(jlo, int(w & 0xffffu), int(w >> 16));
         commit(lds);
         __syncthreads();
@@ -322,17 +361,25 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
         const float2 cs1 = csq[j], cs2 = csq[j + A.h];
         const float c = cs1.x, s = cs1.y, c2 = cs2.x, s2v = cs2.y;
         const unsigned zm = zmask[j - jlo];
+        const unsigned fm = wt[j - jlo].y;
         int s1 = 0, s2 = 0;
+        const float *buf = lds;
+        int view_off = lds_view;  // LDS offset of view j + h's window from view j's
         if constexpr (!G) {
-            const unsigned w = wt[j - jlo];
+            const unsigned w = wt[j - jlo].x;
             s1 = int(w & 0xffffu);
             s2 = int(w >> 16);
-            // prefetch pair j + 1 into registers (the last pair is fetched twice)
-            const int jn = min(j + 1, jhi - 1);
-            const unsigned wn = wt[jn - jlo];
-            fetch(jn, int(wn & 0xffffu), int(wn >> 16));
+            if constexpr (D) {
+                const int sa = int(first_elem(j, s1) & 3), sb = int(first_elem(j + A.h, s2) & 3);
+                buf = lds + sa;
+                view_off = 4 * Up + sb - sa;
+            } else {
+                // prefetch pair j + 1 into registers (the last pair is fetched twice)
+                const int jn = min(j + 1, jhi - 1);
+                const unsigned wn = wt[jn - jlo].x;
+                fetch(jn, int(wn & 0xffffu), int(wn >> 16));
+            }
         }
-        const float *buf = lds;
         // z-independent terms of the pair (Backprojector.cu:53-62)
         const float phat = -xc * c - yc * s;
         const float u = phat * A.inv_dso;
@@ -347,7 +394,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
         const int l1 = min(max(int(fp1) + 1 - s1, 0), W - 2);
         const int l2 = min(max(int(fp2) + 1 - s2, 0), W - 2);
         const float *b1 = buf + __mul24(l1, lp);
-        const float *b2 = buf + lds_view + __mul24(l2, lp);
+        const float *b2 = buf + view_off + __mul24(l2, lp);
         // gather: channel slot = channel + 1 (slots 0, ns + 1 and ns + 2 are zero)
         const T *g1 = Q + size_t(qoff + j) * A.view_stride + size_t(int(fp1) + 1) * rp;
         const T *g2 = Q + size_t(qoff + j + A.h) * A.view_stride + size_t(int(fp2) + 1) * rp;
@@ -361,32 +408,42 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
         // detector row offsets qh(z) = (zc(z) - zr) k1 as z K1 + O1 with zc(z) = (z - wz) dz:
         // the slice index is a literal, so no per-slice coordinate is held in a register
         const float K1 = k1 * A.dz, K2 = k2 * A.dz;
-        const float O1 = fmaf(-A.wz, K1, -zr * k1), O2 = fmaf(-A.wz, K2, -zr2 * k2);
-        // row coordinate in slot units (row + kRowPad), clamped to the guard
-        // slots [kRowPad - 1, nt + kRowPad]: positive, so truncation is floor
+        // and the row coordinates in slot units (row + kRowPad) as r(z) = z K1 + R1,
+        // R1 = O1 + row0, row0 = wt + kRowPad the slot of qh = 0
         const float row0 = A.wt + float(kRowPad);
+        const float R1 = fmaf(-A.wz, K1, fmaf(-zr, k1, row0));
+        const float R2 = fmaf(-A.wz, K2, fmaf(-zr2, k2, row0));
+        // off the detector, a row coordinate is clamped to the guard slots
+        // [kRowPad - 1, nt + kRowPad]: positive, so truncation is floor
         const float lo_r = float(kRowPad) - 1.0f, hi_r = A.nt_f + float(kRowPad);
 #pragma unroll
         for (int z = 0; z < 32; z++) {
             if (zm & (1u << z)) {  // pair inside this slice's window
-                const float qh = fmaf(float(z), K1, O1);
-                const float qh2 = fmaf(float(z), K2, O2);
-                const bool out1 = fabsf(qh) > A.wt, out2 = fabsf(qh2) > A.wt;
-                float wb = -qh2 * __builtin_amdgcn_rcpf(qh - qh2);
-                // keeps wb computed on every lane, so the selects below stay
+                float r1 = fmaf(float(z), K1, R1);
+                float r2 = fmaf(float(z), K2, R2);
+                // conjugate weights (w1, 1 - w1), w1 = qh2 / (qh2 - qh) with qh = r - row0
+                float w1 = (row0 - r2) * __builtin_amdgcn_rcpf(r1 - r2);
+                // keeps w1 computed on every lane, so the selects below stay
                 // selects instead of divergent branches (emits no instruction)
-                asm volatile("" : "+v"(wb));
-                // conjugate weights (w1, 1 - w1); both rays off the detector: 1/2 each
-                // when every lane has both rays on the detector (a wave-uniform
-                // test on the compare masks), the row coordinates lie in the
-                // detector's slots and w1 = wb: the clamps and selects are skipped
-                float w1 = wb, r1 = qh + row0, r2 = qh2 + row0;
-                if (__builtin_amdgcn_ballot_w64(out1 || out2) != 0) {
-                    // keeps this a branch: not speculated into selects (no instruction)
+                asm volatile("" : "+v"(w1));
+                // fm bit z (wave-uniform, from the window table): every column
+                // of the tile has both rays on the detector, so the row
+                // coordinates lie in the detector's slots and the detector
+                // tests, clamps and selects are skipped
+                if (!(fm & (1u << z))) {
+                    // keeps these branches: not speculated into selects (no instruction)
                     asm volatile("");
-                    w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : w1);
-                    r1 = __builtin_amdgcn_fmed3f(r1, lo_r, hi_r);
-                    r2 = __builtin_amdgcn_fmed3f(r2, lo_r, hi_r);
+                    // a ray off the detector (|qh| > wt) takes weight 0, or 1/2
+                    // each when both are; when the compare masks show every
+                    // lane of the wave with both rays on the detector, the
+                    // clamps and selects are skipped
+                    const bool out1 = fabsf(r1 - row0) > A.wt, out2 = fabsf(r2 - row0) > A.wt;
+                    if (__builtin_amdgcn_ballot_w64(out1 || out2) != 0) {
+                        asm volatile("");
+                        w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : w1);
+                        r1 = __builtin_amdgcn_fmed3f(r1, lo_r, hi_r);
+                        r2 = __builtin_amdgcn_fmed3f(r2, lo_r, hi_r);
+                    }
                 }
                 const int i1 = int(r1), i2 = int(r2);
                 const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
@@ -421,7 +478,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
                 acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
             }
         }
-        if constexpr (!G) {
+        if constexpr (D) {
+            __syncthreads();
+            if (j + 1 < jhi) {
+                const unsigned wn = wt[j + 1 - jlo].x;
+                dma(j + 1, int(wn & 0xffffu), int(wn >> 16));
+            }
+            // the copies count as vector-memory loads: this wave waits for
+            // its own, the barrier for every wave's
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        } else if constexpr (!G) {
             // one buffer: every wave is done with pair j before pair j + 1
             // overwrites it (half the LDS of double buffering, so five
             // workgroups fit a CU instead of four)
@@ -438,12 +505,47 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void bp_fold_kernel
     }
 }
 
-// First channel slots of the LDS windows of every (block, tile, pair): the
-// tile centre's channel coordinate through view j (first window) and its
-// mirror 2 ws - c through view j + h (second window).
+// Detector row offsets qh(z) = z K + O of the two rays of pair j through
+// voxel column (x, y) (fractional voxel indices), from the backprojector's
+// per-pair terms (Backprojector.cu:53-70): ray 1 through view j (a = cos,
+// sin), ray 2 through view j + h (b).  Hardware reciprocal and square root:
+// the result only places slices against a margin of a quarter row.
+__device__ __forceinline__ void pair_row_lines(const FoldBpArgs &A, float x, float y, float2 a,
+                                               float2 b, int j, float &K1, float &O1, float &K2,
+                                               float &O2)
+{
+    const float xc = (x - A.wx) * A.dx, yc = (y - A.wy) * A.dy;
+    const float phat = -xc * a.x - yc * a.y;
+    const float u = phat * A.inv_dso;
+    const float theta = A.asin_poly ? asin_poly(u) : asinf(u);
+    const float root = __builtin_amdgcn_sqrtf(A.dso2 - phat * phat);
+    const float lhat = root + xc * a.y - yc * a.x;
+    const float lhat2 = root + xc * b.y - yc * b.x;
+    const float tz = theta * A.inv_orbit;
+    const float zr = A.z0 + (float(j) + tz) * A.dzv;
+    const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
+    const float k1 = A.kc * __builtin_amdgcn_rcpf(lhat), k2 = A.kc * __builtin_amdgcn_rcpf(lhat2);
+    K1 = k1 * A.dz;
+    K2 = k2 * A.dz;
+    O1 = -A.wz * K1 - zr * k1;
+    O2 = -A.wz * K2 - zr2 * k2;
+}
+
+// Per (block, tile, pair):
+//  .x  first channel slots of the LDS windows: the tile centre's channel
+//      coordinate through view j (first window) and its mirror 2 ws - c
+//      through view j + h (second window), s1 | s2 << 16;
+//  .y  bit z set when every voxel column of the tile has both rays of the
+//      pair on the detector in slice z, with a quarter row to spare.
+// qh is linear in z for each column, and smooth over the tile: a quadratic
+// c4 x (1 - x) + c5 y (1 - y) away from the bilinear interpolation of its
+// corner values (x, y in [0, 1] across the tile), so at most |c4|/4 + |c5|/4
+// = dx + dy away, dx and dy its departures at the midpoints of two edges.
+// With e = 2 (dx + dy) for the higher-order terms, |qh| <= wt - 1/4 - e at
+// the four corners holds |qh| <= wt - 1/4 at every column.
 __global__ __launch_bounds__(256) void bp_windows_kernel(FoldBpArgs A, int n_blocks,
                                                          const float2 *__restrict__ cs,
-                                                         unsigned *__restrict__ win)
+                                                         uint2 *__restrict__ win)
 {
     const int tiles = A.n_tiles_x * A.n_tiles_y;
     const long n = long(n_blocks) * tiles * A.n_pairs;
@@ -462,14 +564,61 @@ __global__ __launch_bounds__(256) void bp_windows_kernel(FoldBpArgs A, int n_blo
     const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
     const int s1 = window_start(c1, A.H, A.cp, A.W);
     const int s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, A.W);
-    win[i] = unsigned(s1) | (unsigned(s2) << 16);
+
+    // qh(z) = z K + O of both rays at the corners [a][b] and two edge midpoints
+    const float2 c2 = cs[st[0] + j + A.h];
+    const float x0 = float(tx0), x1 = float(tx0 + kTile - 1);
+    const float y0 = float(ty0), y1 = float(ty0 + kTileY - 1);
+    float K[2][2][2], O[2][2][2], Kx[2], Ox[2], Ky[2], Oy[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+            pair_row_lines(A, a ? x1 : x0, b ? y1 : y0, c, c2, j, K[a][b][0], O[a][b][0], K[a][b][1],
+                           O[a][b][1]);
+    pair_row_lines(A, 0.5f * (x0 + x1), y0, c, c2, j, Kx[0], Ox[0], Kx[1], Ox[1]);
+    pair_row_lines(A, x0, 0.5f * (y0 + y1), c, c2, j, Ky[0], Oy[0], Ky[1], Oy[1]);
+    float e = 0.0f;
+#pragma unroll
+    for (int v = 0; v < 2; v++) {
+        // departures at z = 0 and z = 31 (linear in z between them)
+        const float dx0 = fabsf(Ox[v] - 0.5f * (O[0][0][v] + O[1][0][v]));
+        const float dx1 = fabsf(fmaf(31.0f, Kx[v], Ox[v]) -
+                                0.5f * (fmaf(31.0f, K[0][0][v], O[0][0][v]) + fmaf(31.0f, K[1][0][v], O[1][0][v])));
+        const float dy0 = fabsf(Oy[v] - 0.5f * (O[0][0][v] + O[0][1][v]));
+        const float dy1 = fabsf(fmaf(31.0f, Ky[v], Oy[v]) -
+                                0.5f * (fmaf(31.0f, K[0][0][v], O[0][0][v]) + fmaf(31.0f, K[0][1][v], O[0][1][v])));
+        e = fmaxf(e, 2.0f * (fmaxf(dx0, dx1) + fmaxf(dy0, dy1)));
+    }
+    const float L = A.wt - 0.25f - e;
+    float zlo = -1e30f, zhi = 1e30f;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                // slices with -L <= z K + O <= L (K != 0: dz > 0 and k != 0)
+                const float r = __builtin_amdgcn_rcpf(K[a][b][v]), o = O[a][b][v];
+                const float za = (-L - o) * r, zb = (L - o) * r;
+                zlo = fmaxf(zlo, fminf(za, zb));
+                zhi = fminf(zhi, fmaxf(za, zb));
+            }
+    unsigned fast = 0;
+    if (L > 0.0f) {
+        // a slice within 1e-3 of an end of the interval is left out
+        const int z0 = int(ceilf(fminf(fmaxf(zlo + 1e-3f, 0.0f), 32.0f)));
+        const int z1 = int(floorf(fmaxf(fminf(zhi - 1e-3f, 31.0f), -1.0f)));
+        for (int z = z0; z <= z1; z++) fast |= 1u << z;
+    }
+    win[i] = make_uint2(unsigned(s1) | (unsigned(s2) << 16), fast);
 }
 
 // Device tables of the pair-folded backprojector (see bp_fold_kernel).
 struct BpTables {
     const float2 *cs;
     const unsigned *zmask;
-    const unsigned *win;  // unused by the gather path
+    const uint2 *win;     // channel windows (unused by the gather path) and fast slices
 };
 
 template <typename T>
@@ -502,7 +651,7 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTab
     if (npre < 1 || npre > 8)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window too large");
     // LDS pitches of the 16-, 64- and 96-row detectors get immediate offsets
-    using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const unsigned *);
+    using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const uint2 *);
     static const Kern table[4][8] = {
 #define LCT_ROW(LP)                                                                              \
     {bp_fold_kernel<T, 1, LP>, bp_fold_kernel<T, 2, LP>, bp_fold_kernel<T, 3, LP>,               \
@@ -513,6 +662,31 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTab
     };
     const int row = a.rp == 24 ? 1 : a.rp == 72 ? 2 : a.rp == 104 ? 3 : 0;
     const Kern k = table[row][npre - 1];
+    if (shm > 64 * 1024)
+        LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
+    hipLaunchKernelGGL(k, dim3(unsigned(a.n_wg)), dim3(kBpThreads), shm, ctx->stream, a, tb.cs, tb.zmask,
+                       tb.win);
+    LCT_CHECK_HIP(hipGetLastError());
+    return LCT_OK;
+}
+
+// f32 storage: windows copied by LDS-DMA, Q with the odd pitch rp = lp.
+static int launch_bp_fold_dma(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
+{
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
+    a.lp = a.rp;
+    // a window of W rp floats starting up to 3 floats into its first unit
+    a.units = (a.W * a.rp + 3 + 3) / 4;
+    const size_t shm = size_t(2) * ((a.units + 63) & ~63) * 16;
+    if (shm > 160 * 1024)
+        return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
+    using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const uint2 *);
+    const Kern k = a.rp == 25    ? bp_fold_kernel<float, 1, 25, false, true>
+                   : a.rp == 73  ? bp_fold_kernel<float, 1, 73, false, true>
+                   : a.rp == 105 ? bp_fold_kernel<float, 1, 105, false, true>
+                                 : bp_fold_kernel<float, 1, 0, false, true>;
     if (shm > 64 * 1024)
         LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
@@ -660,19 +834,24 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
 
     // Q: every view rebinned (fused pre-weighting) and filtered once, in the
     // backprojector's layout and storage precision.
-    const int cp = g.ns + 3, rp = g.nt + 8;
+    // f32 windows are copied into LDS as they lie in Q, so Q carries the
+    // backprojector's odd LDS pitch (one more zero row slot); 16-bit windows
+    // are converted on the way and keep rp = nt + 8.
+    const bool dma = o.storage == LCT_F32 && o.interp != LCT_INTERP_GATHER;
+    const int cp = g.ns + 3, rp = g.nt + (dma ? 9 : 8);
     const size_t esz = o.storage == LCT_F32 ? 4 : 2;
     const size_t q_view = size_t(cp) * rp * esz;  // bytes of one view in Q
-    LCT_CHECK(scratch(ctx, kScratchB, q_view * R.nviews, &d_q));
+    // the copies of the last view's window read up to 24 bytes past it
+    const size_t q_bytes = q_view * R.nviews + 256;
+    LCT_CHECK(scratch(ctx, kScratchB, q_bytes, &d_q));
     // The filters write every channel and row slot of the views they fill and
     // never the guard slots, so Q's guards stay zero from one call to the
     // next: Q is cleared only when its buffer, its layout or its extent
     // changed since it was last cleared.
     lct_ctx::QZero &qz = ctx->qzero;
-    if (qz.ptr != d_q || qz.cp != cp || qz.rp != rp || qz.esz != esz ||
-        qz.bytes < q_view * R.nviews) {
-        LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_view * R.nviews, ctx->stream));
-        qz = lct_ctx::QZero{d_q, cp, rp, esz, q_view * R.nviews};
+    if (qz.ptr != d_q || qz.cp != cp || qz.rp != rp || qz.esz != esz || qz.bytes < q_bytes) {
+        LCT_CHECK_HIP(hipMemsetAsync(d_q, 0, q_bytes, ctx->stream));
+        qz = lct_ctx::QZero{d_q, cp, rp, esz, q_bytes};
     }
     // rebin and filter of the resident views [v0, v1) on the context stream
     auto prep = [&](long v0, long v1) -> int {
@@ -742,10 +921,8 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     const int n_tiles = ((ig.nx + kTile - 1) / kTile) * ((ig.ny + kTileY - 1) / kTileY);
     const long n_win = long(nb) * n_tiles * n_pairs;
     void *d_win = nullptr;
-    if (o.interp != LCT_INTERP_GATHER)
-        LCT_CHECK(scratch(ctx, kScratchMisc0, size_t(n_win) * 4, &d_win));
-    const BpTables tb{reinterpret_cast<const float2 *>(d_cs), d_zm,
-                      static_cast<const unsigned *>(d_win)};
+    LCT_CHECK(scratch(ctx, kScratchMisc0, size_t(n_win) * 8, &d_win));
+    const BpTables tb{reinterpret_cast<const float2 *>(d_cs), d_zm, static_cast<const uint2 *>(d_win)};
 
     FoldBpArgs a{};
     a.q = d_q;
@@ -786,7 +963,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         FoldBpArgs b = a;
         b.slab_tab = d_st + 34 * k0;
         BpTables tk = tb;  // the launch numbers its blocks from 0
-        if (tk.win) tk.win += size_t(k0) * n_tiles * n_pairs;
+        tk.win += size_t(k0) * n_tiles * n_pairs;
         LCT_CHECK(tic());
         if (o.interp == LCT_INTERP_GATHER) {
             if (o.storage == LCT_F32)
@@ -796,7 +973,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
             else
                 LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, b, k1 - k0, tk));
         } else if (o.storage == LCT_F32)
-            LCT_CHECK(launch_bp_fold<float>(ctx, b, k1 - k0, tk));
+            LCT_CHECK(launch_bp_fold_dma(ctx, b, k1 - k0, tk));
         else if (o.storage == LCT_F16)
             LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0, tk));
         else
@@ -809,13 +986,11 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         LCT_CHECK(upload(ctx, d_cs, cs_host.data(), cs_host.size() * 4));
         LCT_CHECK(upload(ctx, d_st, tab.data(), tab.size() * 4));
         LCT_CHECK(upload(ctx, d_zm, zmask.data(), zmask.size() * 4));
-        if (d_win) {
-            if (n_win > 0x7fffffffL * 256L)
-                return set_error(LCT_ERR_UNSUPPORTED, "backprojector: window table too large");
-            hipLaunchKernelGGL(bp_windows_kernel, dim3(unsigned((n_win + 255) / 256)), dim3(256), 0,
-                               ctx->stream, a, nb, tb.cs, static_cast<unsigned *>(d_win));
-            LCT_CHECK_HIP(hipGetLastError());
-        }
+        if (n_win > 0x7fffffffL * 256L)
+            return set_error(LCT_ERR_UNSUPPORTED, "backprojector: window table too large");
+        hipLaunchKernelGGL(bp_windows_kernel, dim3(unsigned((n_win + 255) / 256)), dim3(256), 0,
+                           ctx->stream, a, nb, tb.cs, static_cast<uint2 *>(d_win));
+        LCT_CHECK_HIP(hipGetLastError());
         return LCT_OK;
     };
     ctx->last = lct_ctx::LastBlock{2, o.storage, g.n_angle, g.nt, g.ns, cp, rp,
