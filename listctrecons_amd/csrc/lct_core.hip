@@ -150,6 +150,17 @@ void host_taps(int window, int ns, float ds_f, float *h)
     }
 }
 
+int aux_init(lct_ctx *ctx)
+{
+    if (ctx->aux_ready) return LCT_OK;
+    for (auto &s : ctx->aux)
+        if (!s) LCT_CHECK_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (auto &e : ctx->sync_ev)
+        if (!e) LCT_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ctx->aux_ready = true;
+    return LCT_OK;
+}
+
 }  // namespace lct
 
 using namespace lct;
@@ -191,6 +202,13 @@ int lct_ctx_destroy(lct_ctx *ctx)
     if (!ctx) return LCT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    for (auto &s : ctx->aux)
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    for (auto &e : ctx->sync_ev)
+        if (e) (void)hipEventDestroy(e);
     staging_release(ctx);
     for (auto &b : ctx->scratch)
         if (b.ptr) (void)hipFree(b.ptr);

@@ -30,6 +30,13 @@ struct lct_ctx {
     hipEvent_t ev[8] = {};
     bool events_ready = false;
 
+    // Streams and events of the overlapped resident pipeline (later views
+    // rebinned and filtered while the blocks of earlier ones are
+    // backprojected): non-blocking, default priority, created on first use.
+    hipStream_t aux[2] = {nullptr, nullptr};
+    hipEvent_t sync_ev[12] = {};
+    bool aux_ready = false;
+
     // Pinned staging ring for host tables (two halves of kStagingHalf bytes):
     // an upload copies the table into the current half and enqueues an
     // asynchronous copy from there, so the host never waits for the device.
@@ -86,6 +93,8 @@ void clear_error();
 // Returns a device buffer of at least `bytes` in slot `slot` (grown, never shrunk).
 int scratch(lct_ctx *ctx, int slot, size_t bytes, void **out);
 int noalloc_scratch_init(int device);
+// Creates ctx->aux and ctx->sync_ev on first use.
+int aux_init(lct_ctx *ctx);
 
 // Host-to-device copy on the context stream, ordered after earlier work on it;
 // returns once `h` has been read (into the pinned staging ring), without
