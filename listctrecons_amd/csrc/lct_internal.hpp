@@ -25,7 +25,7 @@ struct lct_ctx {
         void *ptr = nullptr;
         size_t bytes = 0;
     };
-    std::vector<Buf> scratch = std::vector<Buf>(8);
+    std::vector<Buf> scratch = std::vector<Buf>(9);
 
     hipEvent_t ev[8] = {};
     bool events_ready = false;
@@ -84,6 +84,7 @@ enum ScratchSlot {
     kScratchFan,    // fan window (per-block pipeline)
     kScratchMisc0,
     kScratchMisc1,
+    kScratchPart,   // second partial sums of the backprojector's split pair ranges
 };
 
 int set_error(int code, const std::string &msg);

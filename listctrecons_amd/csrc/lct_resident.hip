@@ -42,6 +42,8 @@ constexpr int kRowPad = 4;  // zero row slots before row 0
 constexpr int kTile = 16;
 constexpr int kTileY = 32;
 constexpr int kBpThreads = kTile * kTileY;
+// most pair ranges a (block, tile) is split into
+constexpr int kMaxParts = 4;
 
 template <typename T> struct Cvt;
 template <> struct Cvt<float> {
@@ -152,6 +154,10 @@ struct FoldBpArgs {
                          // different banks)
     int asin_poly;       // every voxel column has |p_hat| / dso <= 0.7: asin_poly is valid
     int units;           // D kernels: 16-byte units copied per view window (see bp_fold_kernel)
+    int n_parts;         // pairs of a block in n_parts ranges [jlo + split[p], jlo + split[p + 1])
+    int split[kMaxParts + 1];
+    float *part;         // sums of ranges 1 .. n_parts - 1, one call volume each (range 0: vol)
+    size_t part_stride;  // floats per call volume
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
     int n_pairs;               // pairs j of a block: a_middle[0] - h <= j < a_middle[31]
     float *vol;
@@ -267,8 +273,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
     int wg = blockIdx.x;
     if (A.xcd_remap) wg = (wg & 7) * (A.n_wg >> 3) + (wg >> 3);
     const int tiles = A.n_tiles_x * A.n_tiles_y;
-    const int blk = wg / tiles;
-    const int t = wg - blk * tiles;
+    const int blk = wg / (tiles * A.n_parts);
+    const int rem = wg - blk * tiles * A.n_parts;
+    const int part = rem / tiles;  // which range of the block's pairs
+    const int t = rem - part * tiles;
     const int tx0 = (t % A.n_tiles_x) * kTile, ty0 = (t / A.n_tiles_x) * kTileY;
     const int ix = tx0 + int(threadIdx.x & (kTile - 1));
     const int iy = ty0 + int(threadIdx.x / kTile);
@@ -283,8 +291,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
 #pragma unroll
     for (int z = 0; z < 32; z++) acc[z] = 0.0f;
 
-    // pairs of any slice's window: a_middle[0] - h <= j < a_middle[31]
+    // pairs of any slice's window: a_middle[0] - h <= j < a_middle[31]; this
+    // workgroup takes [jb, je) of them (the tables are indexed from jlo)
     const int jlo = st[2] - A.h, jhi = st[33];
+    const int jb = jlo + A.split[part];
+    const int je = part == A.n_parts - 1 ? jhi : jlo + A.split[part + 1];
 
     uint4 pre[NPRE];
     // Chunk slot i of this thread: chunk index c = tid + kBpThreads i, view
@@ -347,17 +358,17 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
     const float2 *csq = cs + qoff;
     const uint2 *wt = win + size_t(blk * tiles + t) * A.n_pairs;  // index j - jlo
     if constexpr (D) {
-        const unsigned w = wt[0].x;
-        dma(jlo, int(w & 0xffffu), int(w >> 16));
+        const unsigned w = wt[jb - jlo].x;
+        dma(jb, int(w & 0xffffu), int(w >> 16));
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     } else if constexpr (!G) {
-        const unsigned w = wt[0].x;
-        fetch(jlo, int(w & 0xffffu), int(w >> 16));
+        const unsigned w = wt[jb - jlo].x;
+        fetch(jb, int(w & 0xffffu), int(w >> 16));
         commit(lds);
         __syncthreads();
     }
-    for (int j = jlo; j < jhi; j++) {
+    for (int j = jb; j < je; j++) {
         const float2 cs1 = csq[j], cs2 = csq[j + A.h];
         const float c = cs1.x, s = cs1.y, c2 = cs2.x, s2v = cs2.y;
         const unsigned zm = zmask[j - jlo];
@@ -375,7 +386,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
                 view_off = 4 * Up + sb - sa;
             } else {
                 // prefetch pair j + 1 into registers (the last pair is fetched twice)
-                const int jn = min(j + 1, jhi - 1);
+                const int jn = min(j + 1, je - 1);
                 const unsigned wn = wt[jn - jlo].x;
                 fetch(jn, int(wn & 0xffffu), int(wn >> 16));
             }
@@ -480,7 +491,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
         }
         if constexpr (D) {
             __syncthreads();
-            if (j + 1 < jhi) {
+            if (j + 1 < je) {
                 const unsigned wn = wt[j + 1 - jlo].x;
                 dma(j + 1, int(wn & 0xffffu), int(wn >> 16));
             }
@@ -498,11 +509,28 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
         }
     }
     if (ix < A.nx && iy < A.ny) {
-        float *o = A.vol + (size_t(slice0) * A.ny + iy) * A.nx + ix;
+        // several ranges: unscaled sums, added and scaled by bp_sum_parts_kernel
+        float *o = (part ? A.part + size_t(part - 1) * A.part_stride : A.vol) +
+                   (size_t(slice0) * A.ny + iy) * A.nx + ix;
+        const float sc = A.n_parts == 1 ? A.scale : 1.0f;
         const size_t plane = size_t(A.nx) * A.ny;
 #pragma unroll
-        for (int z = 0; z < 32; z++) o[size_t(z) * plane] = acc[z] * A.scale;
+        for (int z = 0; z < 32; z++) o[size_t(z) * plane] = acc[z] * sc;
     }
+}
+
+// vol = (vol + part[0] + ... + part[n_more - 1]) * scale over n floats, left
+// to right: the pair ranges' sums of the backprojector in a fixed order
+__global__ __launch_bounds__(256) void bp_sum_parts_kernel(float *__restrict__ vol,
+                                                           const float *__restrict__ part,
+                                                           size_t stride, int n_more, size_t n,
+                                                           float scale)
+{
+    const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    float v = vol[i];
+    for (int p = 0; p < n_more; p++) v += part[size_t(p) * stride + i];
+    vol[i] = v * scale;
 }
 
 // Detector row offsets qh(z) = z K + O of the two rays of pair j through
@@ -624,7 +652,7 @@ struct BpTables {
 template <typename T>
 static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
-    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks * a.n_parts;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     a.lp = a.rp + 1;
     a.W = 2;  // no LDS window
@@ -637,7 +665,7 @@ static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, cons
 template <typename T>
 static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
-    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks * a.n_parts;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     constexpr int CE = Chunk<T>::kElems;
     a.lp = a.rp + 1;
@@ -674,7 +702,7 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTab
 // f32 storage: windows copied by LDS-DMA, Q with the odd pitch rp = lp.
 static int launch_bp_fold_dma(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
-    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks;
+    a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks * a.n_parts;
     a.xcd_remap = (a.n_wg % 8 == 0) ? 1 : 0;
     a.lp = a.rp;
     // a window of W rp floats starting up to 3 floats into its first unit
@@ -958,6 +986,44 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.ns_f = float(g.ns);
     a.nt_f = float(g.nt);
     a.scale = float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
+    // Each (block, tile) can be backprojected by kParts workgroups, one per
+    // range of the block's pairs (ranges of about equal work: a pair costs
+    // about 4 slices' worth of set-up and staging plus its active slices).
+    // Shorter workgroups shorten the launch's tail, where slots empty one by
+    // one behind the last workgroups to start (on the clinical volume the
+    // whole GPU idled ~3 ms there out of 29). The ranges are the same for
+    // every block, so a block's sums do not depend on which blocks share its
+    // launch, and a slab equals the same blocks of the whole volume bit for
+    // bit. Measured (whole volumes back to back, same box): 16-bit storage
+    // 32.1 -> 31.2 ms with 4 ranges (2: 32.0, 3: 31.5, 6: 31.2, 8: 31.3);
+    // the large volume, 42 rounds of workgroups, 410 -> 409 ms; f32 storage
+    // (the LDS-DMA kernel, 4 workgroups per CU) 39.0 -> 41.4 ms, so it keeps
+    // one range.
+    const int kParts = 4;
+    const bool want_parts = o.storage != LCT_F32 && o.interp != LCT_INTERP_GATHER;
+    a.n_parts = 1;
+    a.part = nullptr;
+    a.part_stride = size_t(nb) * LCT_BLOCK_SLICES * ig.ny * ig.nx;
+    for (int &x : a.split) x = 0;
+    if (want_parts && n_pairs >= 2 * kParts) {
+        long total = 0, run = 0;
+        for (unsigned m : zmask) total += m ? __builtin_popcount(m) + 4 : 0;
+        int p = 1;
+        for (int jj = 0; jj < n_pairs && p < kParts; jj++) {
+            run += zmask[jj] ? __builtin_popcount(zmask[jj]) + 4 : 0;
+            if (run * kParts >= total * p) a.split[p++] = jj + 1;
+        }
+        bool ok = p == kParts;
+        for (int q = 1; q < kParts && ok; q++) ok = a.split[q] > a.split[q - 1] && a.split[q] < n_pairs;
+        if (ok) {
+            void *d_part = nullptr;
+            LCT_CHECK(scratch(ctx, kScratchPart, (kParts - 1) * a.part_stride * 4, &d_part));
+            a.part = static_cast<float *>(d_part);
+            a.n_parts = kParts;
+        } else {
+            for (int &x : a.split) x = 0;
+        }
+    }
     // backprojection of blocks [k0, k1) in one launch on the context stream
     auto bp = [&](int k0, int k1) -> int {
         FoldBpArgs b = a;
@@ -978,6 +1044,14 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
             LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0, tk));
         else
             LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, b, k1 - k0, tk));
+        if (a.n_parts > 1) {
+            const size_t slab = size_t(LCT_BLOCK_SLICES) * ig.ny * ig.nx;
+            const size_t n = size_t(k1 - k0) * slab;
+            hipLaunchKernelGGL(bp_sum_parts_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                               ctx->stream, d_vol + size_t(k0) * slab, a.part + size_t(k0) * slab,
+                               a.part_stride, a.n_parts - 1, n, a.scale);
+            LCT_CHECK_HIP(hipGetLastError());
+        }
         LCT_CHECK(toc(&t.backproject_ms));
         t.n_backproject_launches++;
         return LCT_OK;

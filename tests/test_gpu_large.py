@@ -102,3 +102,16 @@ def test_large_detector_paths_agree(ctx, large, plain_volume):
         else:
             assert float(d.pow(2).mean().sqrt()) * scan.hu_per_unit() <= 1.0
         del got, d
+
+
+def test_large_detector_slab_equals_the_whole_volume(ctx, large):
+    """A 4-block slab from a buffer holding only its views equals those
+    blocks of the one-shot volume bit for bit (mixed f16: the block's pairs
+    are split over four workgroups the same way in both)."""
+    w, fan, a, c, pl = large
+    opts = recon.preset("mixed_f16", window=w.window)
+    whole = ctx.reconstruct(pl, opts, a, c, fan).clone()
+    v0, v1 = recon.slab_views(pl, c, 4, 8)
+    got = ctx.reconstruct_slabs(pl, opts, a, c, fan[v0:v1].contiguous(), v0, 4, 8)
+    ctx.synchronize()
+    assert torch.equal(got, whole[32 * 4:32 * 8])
