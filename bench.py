@@ -15,9 +15,11 @@ Optional: slab streaming of the clinical workload from pinned host memory
 (--stream-slabs), the learned correction's inference (--no-learned to skip),
 the LISTCTRecons CLI end to end on the workload's files (--cli-e2e).
 
-The roofline of the backprojector is taken against VALU issue, the unit that
-binds it: the committed SQ_INSTS_VALU count of the launch (profiles/) over
-the launch time this run measures with HIP events.
+The roofline of the backprojector is taken against the unit that binds it,
+the VALU: the busy cycles the committed counters of the launch give
+(profiles/: SQ_ACTIVE_INST_VALU less the cycles two VALU instructions shared,
+SQ_ACTIVE_INST_VALU2) over the SIMD-cycles of the kernel's launch time, which
+this run measures with HIP events.
 
 Prints one JSON line (rank 0).
 """
@@ -36,9 +38,16 @@ METRIC = "s/volume + backproj GUPS, 512²×256: FP32 vs mixed/learned at ≤1 HU
 HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 ENGINE_HZ = 2.4e9  # MI355X peak engine clock
 N_CU, N_SIMD = 256, 1024
-# VALU issue peak: a wave64 FP32 instruction occupies a SIMD for 2 cycles
-# (128 FP32 lanes per CU per clock), so 1024 SIMDs x 64 lanes / 2 cycles x
-# 2.4 GHz = 78.6e12 lane-operations/s (157.3 TFLOP/s if every one were an FMA)
+# VALU issue: a wave64 VALU instruction holds its SIMD for one quad-cycle (two
+# for transcendentals), and two waves' FP32 add / mul / fma (and 32-bit
+# add / logic) instructions can share one (SQ_ACTIVE_INST_VALU2 counts those
+# quad-cycles): 2 cycles per instruction at best, 4 for conversions, fract,
+# floor, min / max / med3, shifts, packed and mixed 16-bit arithmetic, 8 for
+# rcp / sqrt (listctrecons_amd/bin/issue_rates measures them).  The VALU is
+# busy 4 (ACTIVE_INST_VALU - ACTIVE_INST_VALU2) cycles per launch out of
+# 1024 SIMDs x 2.4 GHz x the launch time.  At 2 cycles per instruction the
+# lane-operation peak would be 1024 x 64 / 2 x 2.4 GHz = 78.6e12 / s.
+VALU_CYCLES_PEAK = N_SIMD * ENGINE_HZ
 VALU_LANE_OPS_PEAK = N_SIMD * 64 / 2 * ENGINE_HZ
 KERNEL_SOURCE = os.path.join(ROOT, "listctrecons_amd", "csrc", "lct_resident.hip")
 
@@ -64,6 +73,7 @@ def time_path(ctx, pl, opts, a, c, fan, vol, steps, warmup):
     # with HIP events and waits for it
     ctx.set_profiling(True)
     stage = {"rebin_ms": 0.0, "filter_ms": 0.0, "backproject_ms": 0.0}
+    bp_kernel_ms = 0.0
     launches = 0
     per = []
     for _ in range(steps):
@@ -73,6 +83,7 @@ def time_path(ctx, pl, opts, a, c, fan, vol, steps, warmup):
         per.append(time.perf_counter() - s0)
         for k in stage:
             stage[k] += t[k]
+        bp_kernel_ms += t["backproject_kernel_ms"]
         launches += t["n_backproject_launches"]
     torch.cuda.synchronize()
     ctx.set_profiling(False)
@@ -82,6 +93,8 @@ def time_path(ctx, pl, opts, a, c, fan, vol, steps, warmup):
         "median_s": sorted(per)[len(per) // 2],
         "stage_ms": {k: v / steps for k, v in stage.items()},
         "bp_ms_per_launch": stage["backproject_ms"] / max(launches, 1),
+        # the backprojector kernel alone (without the pass adding split blocks)
+        "bp_kernel_ms_per_launch": bp_kernel_ms / max(launches, 1),
         "bp_launches_per_volume": launches / steps,
         "updates": updates,
     }
@@ -290,37 +303,44 @@ def source_sha16(path=KERNEL_SOURCE):
 
 def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_tap, n_vox):
     """Roofline of the backprojector launch against the unit that binds it,
-    VALU issue.  achieved = VALU lane-operations per launch (SQ_INSTS_VALU x 64
-    from the committed counters of this kernel and workload, which depend on
-    the code, not on timing) over the launch time measured in this run with
-    HIP events on the context stream; peak = 78.6e12 lane-ops/s.  So frac =
-    SQ_INSTS_VALU x 2 cycles / 1024 SIMDs / (launch s x 2.4 GHz).  Beside it:
-    LDS-active and bank-conflict fractions (SQ counters), measured HBM traffic
-    (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and its share of the 8 TB/s
-    peak, and the gather-byte rate of the north star's definition (2 rays x 4
-    taps x element size per update: the taps come from LDS, not HBM)."""
+    the VALU.  achieved = the VALU's busy cycles per launch, 4 x
+    (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) quad-cycles from the
+    committed counters of this kernel and workload (they follow the code and
+    the workload, not the timing), over the kernel's launch time measured in
+    this run with HIP events on the context stream; peak = 1024 SIMDs x
+    2.4 GHz of VALU cycles; frac = achieved / peak.  Beside it: the same
+    instructions timed as if each took 2 cycles (`frac_at_2_cycles`, a lower
+    bound), VALU lane-operations per update, LDS-active and bank-conflict
+    fractions, measured HBM traffic (FETCH_SIZE / WRITE_SIZE, copy-calibrated)
+    and its share of the 8 TB/s peak, and the gather-byte rate of the north
+    star's definition (2 rays x 4 taps x element size per update: the taps
+    come from LDS, not HBM)."""
     per_launch_updates = updates / launches
     gather_bytes = (updates * 2 * 4 * bytes_per_tap + n_vox * 4) / launches
     out = {"bound": "valu", "kernel": kernel}
     insts = prof.get("sq_insts_valu_per_launch")
-    if insts:
-        achieved = insts * 64 / launch_s
+    act, act2 = prof.get("sq_active_inst_valu_per_launch"), prof.get("sq_active_inst_valu2_per_launch")
+    cycles = launch_s * ENGINE_HZ
+    if insts and act and act2 is not None:
+        busy = 4.0 * (act - act2)
         out.update({
-            "achieved": round(achieved / 1e12, 2),
-            "peak": round(VALU_LANE_OPS_PEAK / 1e12, 2),
-            "unit": "T VALU lane-ops/s",
-            "frac": round(achieved / VALU_LANE_OPS_PEAK, 4),
+            "achieved": round(busy / launch_s / 1e12, 3),
+            "peak": round(VALU_CYCLES_PEAK / 1e12, 3),
+            "unit": "T VALU SIMD-cycles/s",
+            "frac": round(busy / launch_s / VALU_CYCLES_PEAK, 4),
+            "valu_busy_cycles_per_launch": busy,
             "valu_insts_per_launch": insts,
+            "frac_at_2_cycles": round(insts * 64 / launch_s / VALU_LANE_OPS_PEAK, 4),
             "valu_lane_ops_per_update": round(insts * 64 / per_launch_updates, 2),
         })
     else:
-        out.update({"achieved": None, "peak": round(VALU_LANE_OPS_PEAK / 1e12, 2),
-                    "unit": "T VALU lane-ops/s", "frac": None})
+        out.update({"achieved": None, "peak": round(VALU_CYCLES_PEAK / 1e12, 3),
+                    "unit": "T VALU SIMD-cycles/s", "frac": None})
     traffic = prof.get("traffic_bytes_per_launch")
-    cycles = launch_s * ENGINE_HZ
     out.update({
         "traffic": traffic,
         "traffic_frac_of_hbm_peak": round(traffic / launch_s / HBM_PEAK, 4) if traffic else None,
+        "salu_insts_per_launch": prof.get("sq_insts_salu_per_launch"),
         "lds_busy": (round(prof["sq_lds_idx_active_per_launch"] / N_CU / cycles, 3)
                      if "sq_lds_idx_active_per_launch" in prof else None),
         "lds_bank_conflict": (round(prof["sq_lds_bank_conflict_per_launch"] / N_CU / cycles, 3)
@@ -360,7 +380,7 @@ def large_workload(ctx, args, recon, scan, torch):
            "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"] * 1e-3) / 1e9, 2),
            "roofline": backprojector_roofline(prof, prof.get("kernel", "bp_fold_kernel"),
                                               r["updates"], r["bp_launches_per_volume"],
-                                              r["bp_ms_per_launch"] * 1e-3,
+                                              r["bp_kernel_ms_per_launch"] * 1e-3,
                                               4 if storage == "f32" else 2, n_vox)}
     if args.large_slab > 0:
         host_fan = torch.empty(fan.shape, dtype=fan.dtype, pin_memory=True)
@@ -547,7 +567,7 @@ def main():
                "mixed_f16_gather": "f16", "mixed_bf16_gather": "bf16"}.get(args.path, "f32")
     bytes_per_tap = 2 if storage in ("f16", "bf16") else 4
     updates = head["updates"]
-    bp_s = head["bp_ms_per_launch"] * 1e-3
+    bp_s = head["bp_kernel_ms_per_launch"] * 1e-3
     launches = head["bp_launches_per_volume"]
     gups = updates / (head["stage_ms"]["backproject_ms"] * 1e-3) / 1e9
     prof = load_profile(f"bp_fold_{storage}_{w.name}")

@@ -214,6 +214,9 @@ typedef struct {
     int n_backproject_launches;
     long long backproject_updates; /* voxel-view updates (one per view in a
                                       voxel's window) */
+    float backproject_kernel_ms;   /* of backproject_ms, the backprojector
+                                      launches alone (without the pass that
+                                      adds the partial sums of split blocks) */
 } lct_timings;
 
 /* FDKReconstructor::reconstruct (Src/Reconstructor.cpp:30-49) for the whole

@@ -100,7 +100,7 @@ class Timings(C.Structure):
     _fields_ = [
         ("rebin_ms", C.c_float), ("filter_ms", C.c_float), ("backproject_ms", C.c_float),
         ("total_ms", C.c_float), ("n_backproject_launches", C.c_int),
-        ("backproject_updates", C.c_longlong),
+        ("backproject_updates", C.c_longlong), ("backproject_kernel_ms", C.c_float),
     ]
 
 

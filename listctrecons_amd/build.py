@@ -6,6 +6,7 @@ Outputs (git-ignored, shipped with the tree):
   listctrecons_amd/liblct.so            HIP kernels + C ABI (include/lct.h), gfx950
   listctrecons_amd/libListCTRecons.so   C++ drop-in classes over the C ABI
   listctrecons_amd/bin/LISTCTRecons     the reference CLI (LISTCTRecons <prm>)
+  listctrecons_amd/bin/issue_rates      issue cost of VALU / SALU / LDS instruction classes
   oracle/liblct_oracle.so               FP64 CPU oracle (test infrastructure)
   oracle/_ref/*                         reference sources that compile alone, and the
                                         reference's main.cpp linked to the package
@@ -132,12 +133,32 @@ def build_ref_main() -> None:
         _run(["make", "-B", "-s", "-C", oracle_dir, "-f", "ref.mk", "_ref/ref_main"])
 
 
+TOOLS = os.path.join(PKG, "tools")
+
+
+def build_tools() -> None:
+    """Measurement programs (tools/*.hip) into bin/: issue_rates, the issue
+    cost of the instruction classes the backprojector uses."""
+    if not os.path.isdir(TOOLS):
+        return
+    out_dir = os.path.join(PKG, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    for f in sorted(os.listdir(TOOLS)):
+        if not f.endswith(".hip"):
+            continue
+        exe = os.path.join(out_dir, f[:-4])
+        _run([HIPCC, "-O3", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature",
+              "-Xclang", "-packed-fp32-ops", "-o", exe + ".tmp", os.path.join(TOOLS, f)])
+        os.replace(exe + ".tmp", exe)
+
+
 def build_all(jobs: int = 8) -> None:
     build_oracle()
     build_lct(jobs)
     build_cpp(jobs)
     build_native_tests()
     build_ref_main()
+    build_tools()
 
 
 if __name__ == "__main__":

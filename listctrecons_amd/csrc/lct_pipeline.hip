@@ -101,6 +101,7 @@ static int reconstruct_blocks(lct_ctx *ctx, const lct_plan &plan, const lct_reco
         LCT_CHECK(backproject_block_public(ctx, o.bp_kind, g, ig, par, betas.data(),
                                            d_vol + size_t(b - b0) * slab));
         LCT_CHECK(clk.stop(&t.backproject_ms));
+        t.backproject_kernel_ms = t.backproject_ms;
         t.n_backproject_launches++;
     }
     // the last block's filtered views stay in `par` (CTDataCPU::rebin of the
