@@ -249,6 +249,9 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
     };
     bf16x8_t bnext[NF];
     load_b(0, bnext);
+    // unrolled: every halo and weight address is a lane base plus a constant
+    // (the LDS offsets go into the instructions), no per-tap index arithmetic
+#pragma unroll
     for (int tap = 0; tap < kTaps; tap++) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
         bf16x8_t bfrag[NF];

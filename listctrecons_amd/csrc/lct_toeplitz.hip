@@ -35,7 +35,9 @@
 namespace lct {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kTM = 128;     // channels per workgroup tile (8 fragments of 16)
 constexpr int kTN = 64;      // (view,row) columns per tile (4 fragments of 16)
@@ -52,15 +54,6 @@ struct ToeplitzArgs {
     int ns, nt, cp, rp, kp, n_m_tiles, n_n_wg;
     int xcd_remap;  // grid is a multiple of 8: deal consecutive tiles to one XCD
 };
-
-__device__ __forceinline__ unsigned short bf16_rn(float x)
-{
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-
-__device__ __forceinline__ float bf16_to_f(unsigned short b) { return __uint_as_float(unsigned(b) << 16); }
 
 template <typename T> __device__ __forceinline__ T to_store(float x);
 template <> __device__ __forceinline__ float to_store<float>(float x) { return x; }
@@ -111,20 +104,20 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
             for (int j = 0; j < 8; j++) x[j] = 0.0f;
         }
     };
+    // x = hi + lo: both rounded to nearest even by the packed hardware
+    // conversion (two values per v_cvt_pk_bf16_f32); x - hi is exact in FP32
     auto commit_b = [&](int buf, const float (&x)[8]) {
-        unsigned short hi[8], lo[8];
+        unsigned hi[4], lo[4];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            hi[j] = bf16_rn(x[j]);
-            lo[j] = bf16_rn(x[j] - bf16_to_f(hi[j]));
+        for (int e = 0; e < 4; e++) {
+            const f32x2 v = {x[2 * e], x[2 * e + 1]};
+            const bf16x2 h = __builtin_convertvector(v, bf16x2);
+            const f32x2 r = v - __builtin_convertvector(h, f32x2);
+            hi[e] = __builtin_bit_cast(unsigned, h);
+            lo[e] = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
         }
-        uint4 vh, vl;
-        vh.x = hi[0] | (unsigned(hi[1]) << 16); vh.y = hi[2] | (unsigned(hi[3]) << 16);
-        vh.z = hi[4] | (unsigned(hi[5]) << 16); vh.w = hi[6] | (unsigned(hi[7]) << 16);
-        vl.x = lo[0] | (unsigned(lo[1]) << 16); vl.y = lo[2] | (unsigned(lo[3]) << 16);
-        vl.z = lo[4] | (unsigned(lo[5]) << 16); vl.w = lo[6] | (unsigned(lo[7]) << 16);
-        *reinterpret_cast<uint4 *>(bh + (buf * kTN + bcol) * kBRow + bk) = vh;
-        *reinterpret_cast<uint4 *>(bl + (buf * kTN + bcol) * kBRow + bk) = vl;
+        *reinterpret_cast<uint4 *>(bh + (buf * kTN + bcol) * kBRow + bk) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+        *reinterpret_cast<uint4 *>(bl + (buf * kTN + bcol) * kBRow + bk) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
     };
 
     const long n_tiles = (A.n_cols + kTN - 1) / kTN;
