@@ -63,8 +63,10 @@ template <> __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(f
     return __float2bfloat16(x);
 }
 
+// 4 waves per SIMD (at most 128 registers, the accumulators in VGPRs, no spills)
 template <typename T>
-__global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
+toeplitz_filter_kernel(ToeplitzArgs A)
 {
     __shared__ __attribute__((aligned(16))) unsigned short bh[2 * kTN * kBRow];
     __shared__ __attribute__((aligned(16))) unsigned short bl[2 * kTN * kBRow];
@@ -129,6 +131,22 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
 #pragma unroll
             for (int f = 0; f < 4; f++) acc[a][f] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 
+        // A fragments of this wave's two 16-channel slices at k-step ks: diagonal
+        // index (m - 32 ks + kp - 32) / 16 of the precomputed table. Those of
+        // k-step ks + 1 are loaded while ks runs, so the table's L2 latency is
+        // not waited on at every step.
+        bf16x8 a1[2], a2[2];
+        auto load_a = [&](int ks, bf16x8 (&p1)[2], bf16x8 (&p2)[2]) {
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int m = m0 + (2 * wave + a) * 16;
+                const long di = (m - ks * kTK + A.kp - kTK) / 16;
+                const unsigned short *f = A.frag + (di * 64 + lane) * 8;
+                p1[a] = *reinterpret_cast<const bf16x8 *>(f);
+                p2[a] = *reinterpret_cast<const bf16x8 *>(f + A.frag_stride);
+            }
+        };
+        load_a(0, a1, a2);
         float xb[8];
         stage_b(n0, 0, xb);
         commit_b(0, xb);
@@ -136,17 +154,8 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
         for (int ks = 0; ks < n_ksteps; ks++) {
             const int buf = ks & 1;
             if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, xb);
-            // A fragments of this wave's two 16-channel slices: diagonal index
-            // (m - k0 + kp - 32) / 16 of the precomputed table
-            bf16x8 a1[2], a2[2];
-#pragma unroll
-            for (int a = 0; a < 2; a++) {
-                const int m = m0 + (2 * wave + a) * 16;
-                const long di = (m - ks * kTK + A.kp - kTK) / 16;
-                const unsigned short *f = A.frag + (di * 64 + lane) * 8;
-                a1[a] = *reinterpret_cast<const bf16x8 *>(f);
-                a2[a] = *reinterpret_cast<const bf16x8 *>(f + A.frag_stride);
-            }
+            bf16x8 n1[2], n2[2];
+            load_a(ks + 1 < n_ksteps ? ks + 1 : ks, n1, n2);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const int c = f * 16 + li;
@@ -160,6 +169,11 @@ __global__ __launch_bounds__(256) void toeplitz_filter_kernel(ToeplitzArgs A)
                 }
             }
             if (ks + 1 < n_ksteps) commit_b(buf ^ 1, xb);
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                a1[a] = n1[a];
+                a2[a] = n2[a];
+            }
             __syncthreads();
         }
         // epilogue: D[i][n] -> Q[v][ch + 1][r + 4]
