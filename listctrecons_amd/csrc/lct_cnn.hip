@@ -53,6 +53,14 @@ __device__ __forceinline__ bf16_t f2bf(float x)
     return bf16_t(u >> 16);
 }
 __device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float(unsigned(b) << 16); }
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+// (a, b) -> two bf16 in one word, rounded to nearest even by the hardware
+// conversion (v_cvt_pk_bf16_f32): the same bits as f2bf for finite values
+__device__ __forceinline__ unsigned pack_bf16x2(float a, float b)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2_t{a, b}, bf16x2_t));
+}
 
 struct Shape {
     int B, D, H, W;
@@ -79,6 +87,40 @@ __device__ __forceinline__ bool inside(const Shape &s, int z, int y, int x)
 constexpr int kTZ = 4, kTY = 4, kTX = 16;  // 2 x 16 and 8 x 16 y-x tiles ran slower
 constexpr int kHZ = kTZ + 2, kHY = kTY + 2, kHX = kTX + 2;
 constexpr int kHalo = kHZ * kHY * kHX;  // 648 voxels, 41472 B of LDS
+// Largest plane (H W voxels) the conv kernels take: the offsets of a tile's
+// voxels and halo chunks from the tile's first voxel then fit 32 bits (bytes
+// of a 32-channel bf16 halo: 6 planes x 2^23 x 64 B < 2^32).
+constexpr long kMaxPlane = 1L << 23;
+
+// The conv kernels run on a 3D grid: x = tile column, y = tile row, z = tile
+// plane + tiles_z x batch item, so a workgroup finds its tile without the
+// 64-bit divisions of a flat index.
+struct TileOrg {
+    int b, z0, y0, x0;
+};
+__device__ __forceinline__ TileOrg tile_origin(const Shape &s)
+{
+    const int tiles_z = (s.D + kTZ - 1) / kTZ;
+    const int b = int(blockIdx.z) / tiles_z;
+    return TileOrg{b, (int(blockIdx.z) - b * tiles_z) * kTZ, int(blockIdx.y) * kTY, int(blockIdx.x) * kTX};
+}
+// the (kTZ+2) x (kTY+2) x (kTX+2) halo lies inside the volume (every tile but
+// those at its faces): no clamps or padding, addresses from the tile base
+__device__ __forceinline__ bool halo_inside(const Shape &s, const TileOrg &o)
+{
+    return o.z0 >= 1 && o.z0 + kTZ < s.D && o.y0 >= 1 && o.y0 + kTY < s.H && o.x0 >= 1 &&
+           o.x0 + kTX < s.W;
+}
+// the tile's output voxels all lie inside the volume
+__device__ __forceinline__ bool tile_inside(const Shape &s, const TileOrg &o)
+{
+    return o.z0 + kTZ <= s.D && o.y0 + kTY <= s.H && o.x0 + kTX <= s.W;
+}
+// offset of tile voxel (zl, yl, xl) from the tile's first voxel
+__device__ __forceinline__ unsigned tile_off(const Shape &s, int zl, int yl, int xl)
+{
+    return (unsigned(zl) * unsigned(s.H) + unsigned(yl)) * unsigned(s.W) + unsigned(xl);
+}
 
 // ------------------------------------------- layer 1 (1 -> 32) on MFMA
 // h[v][co] = relu(b[co] + sum_tap w[tap][co] x[v + off(tap)]) as a GEMM with
@@ -93,29 +135,31 @@ __global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__
 {
     __shared__ __attribute__((aligned(16))) float lds[kTZ * kTY * kTX * kC];  // 32 KB
     float *hx_ = lds;  // halo, 648 floats, until the products are done
-    const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
-    const int tiles_z = (s.D + kTZ - 1) / kTZ;
-    long t = blockIdx.x;
-    const int tx = int(t % tiles_x);
-    t /= tiles_x;
-    const int ty = int(t % tiles_y);
-    t /= tiles_y;
-    const int tz = int(t % tiles_z);
-    const int b = int(t / tiles_z);
-    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    const TileOrg o = tile_origin(s);
+    const int b = o.b, z0 = o.z0, y0 = o.y0, x0 = o.x0;
 
     constexpr int kStage = (kHalo + 255) / 256;
     float val[kStage];
+    if (halo_inside(s, o)) {
+        const float *src = x + vox(s, b, z0 - 1, y0 - 1, x0 - 1);
 #pragma unroll
-    for (int i = 0; i < kStage; i++) {
-        const int c = min(int(threadIdx.x) + 256 * i, kHalo - 1);
-        const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
-        const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
-        const bool pad = !inside(s, gz, gy, gx);
-        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
-        const int cx = min(max(gx, 0), s.W - 1);
-        const float v = x[vox(s, b, cz, cy, cx)];
-        val[i] = pad ? 0.0f : v;
+        for (int i = 0; i < kStage; i++) {
+            const int c = min(int(threadIdx.x) + 256 * i, kHalo - 1);
+            const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
+            val[i] = src[tile_off(s, hz, hy, hx)];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kStage; i++) {
+            const int c = min(int(threadIdx.x) + 256 * i, kHalo - 1);
+            const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
+            const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+            const bool pad = !inside(s, gz, gy, gx);
+            const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
+            const int cx = min(max(gx, 0), s.W - 1);
+            const float v = x[vox(s, b, cz, cy, cx)];
+            val[i] = pad ? 0.0f : v;
+        }
     }
 #pragma unroll
     for (int i = 0; i < kStage; i++) {
@@ -133,19 +177,29 @@ __global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__
     f32x4_t acc[kTY][2];
 #pragma unroll
     for (int m = 0; m < kTY; m++) {
-        // A fragment: voxel (wave, m, li), taps lk .. lk + 7
-        bf16x8_t ahi, alo;
+        // A fragment: voxel (wave, m, li), taps lk .. lk + 7, split as
+        // v = hi + lo by the packed hardware conversion (round to nearest even,
+        // two values per v_cvt_pk_bf16_f32; v - hi is exact in FP32)
+        float v[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
             const int tap = lk + j;
-            float v = 0.0f;
+            v[j] = 0.0f;
             if (tap < kTaps) {
                 const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-                v = hx_[((wave + 1 + dz) * kHY + m + 1 + dy) * kHX + li + 1 + dx];
+                v[j] = hx_[((wave + 1 + dz) * kHY + m + 1 + dy) * kHX + li + 1 + dx];
             }
-            const bf16_t hi = f2bf(v);
-            ahi[j] = __builtin_bit_cast(__bf16, hi);
-            alo[j] = __builtin_bit_cast(__bf16, f2bf(v - bf2f(hi)));
+        }
+        bf16x8_t ahi, alo;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const f32x2_t pv = {v[2 * e], v[2 * e + 1]};
+            const bf16x2_t h = __builtin_convertvector(pv, bf16x2_t);
+            const bf16x2_t l = __builtin_convertvector(pv - __builtin_convertvector(h, f32x2_t), bf16x2_t);
+            ahi[2 * e] = h[0];
+            ahi[2 * e + 1] = h[1];
+            alo[2 * e] = l[0];
+            alo[2 * e + 1] = l[1];
         }
 #pragma unroll
         for (int f = 0; f < 2; f++) {
@@ -166,21 +220,27 @@ __global__ __launch_bounds__(256) void conv_in1_kernel(const float *__restrict__
                 tile[vl * kC + f * 16 + li] = acc[m][f][j];
             }
     __syncthreads();
-    for (int c = threadIdx.x; c < kTZ * kTY * kTX * 4; c += 256) {
-        const int vl = c >> 2, part = c & 3;
+    // every pass of a thread writes the same 8 channels (256 is a multiple of 4)
+    const int part = threadIdx.x & 3;
+    float bv[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) bv[e] = bias[part * 8 + e];
+    const bool whole = tile_inside(s, o);
+    uint4 *dst = reinterpret_cast<uint4 *>(h + vox(s, b, z0, y0, x0) * kC) + part;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int vl = (int(threadIdx.x) >> 2) + 64 * k;
         const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
-        const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
-        if (!inside(s, gz, gy, gx)) continue;
+        if (!whole && !inside(s, z0 + zl, y0 + yl, x0 + xl)) continue;
         const float *src = tile + vl * kC + part * 8;
-        unsigned o[4];
+        unsigned w2[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const int ch = part * 8 + 2 * e;
-            const float a0 = fmaxf(src[2 * e] + bias[ch], 0.0f);
-            const float a1 = fmaxf(src[2 * e + 1] + bias[ch + 1], 0.0f);
-            o[e] = unsigned(f2bf(a0)) | (unsigned(f2bf(a1)) << 16);
+            const float a0 = fmaxf(src[2 * e] + bv[2 * e], 0.0f);
+            const float a1 = fmaxf(src[2 * e + 1] + bv[2 * e + 1], 0.0f);
+            w2[e] = pack_bf16x2(a0, a1);
         }
-        reinterpret_cast<uint4 *>(h + vox(s, b, gz, gy, gx) * kC)[part] = make_uint4(o[0], o[1], o[2], o[3]);
+        dst[size_t(tile_off(s, zl, yl, xl)) * 4] = make_uint4(w2[0], w2[1], w2[2], w2[3]);
     }
 }
 
@@ -195,34 +255,39 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 {
     constexpr int NF = MODE == 2 ? 1 : 2;  // N fragments of 16 output channels
     __shared__ __attribute__((aligned(16))) bf16_t halo[kHalo * kC];
-    const int tiles_x = (s.W + kTX - 1) / kTX, tiles_y = (s.H + kTY - 1) / kTY;
-    const int tiles_z = (s.D + kTZ - 1) / kTZ;
-    long t = blockIdx.x;
-    const int tx = int(t % tiles_x);
-    t /= tiles_x;
-    const int ty = int(t % tiles_y);
-    t /= tiles_y;
-    const int tz = int(t % tiles_z);
-    const int b = int(t / tiles_z);
-    const int x0 = tx * kTX, y0 = ty * kTY, z0 = tz * kTZ;
+    const TileOrg o = tile_origin(s);
+    const int b = o.b, z0 = o.z0, y0 = o.y0, x0 = o.x0;
 
     // stage the halo: 648 voxels x 4 chunks of 16 bytes, every load of a
     // thread issued before its LDS writes (one memory latency, not eleven)
-    // (loads from clamped, always valid addresses; the zero padding is
-    // applied when the chunk is written)
     constexpr int kStage = (kHalo * 4 + 255) / 256;
     uint4 val[kStage];
     bool pad[kStage];
+    if (halo_inside(s, o)) {
+        // chunk (hz, hy, hx, part) at a 32-bit byte offset from the halo's first voxel
+        const char *src = reinterpret_cast<const char *>(in + vox(s, b, z0 - 1, y0 - 1, x0 - 1) * kC);
 #pragma unroll
-    for (int i = 0; i < kStage; i++) {
-        const int c = min(int(threadIdx.x) + 256 * i, kHalo * 4 - 1);
-        const int hv = c >> 2, part = c & 3;
-        const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
-        const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
-        pad[i] = !inside(s, gz, gy, gx);
-        const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
-        const int cx = min(max(gx, 0), s.W - 1);
-        val[i] = reinterpret_cast<const uint4 *>(in + vox(s, b, cz, cy, cx) * kC)[part];
+        for (int i = 0; i < kStage; i++) {
+            const int c = min(int(threadIdx.x) + 256 * i, kHalo * 4 - 1);
+            const int hv = c >> 2, part = c & 3;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            pad[i] = false;
+            val[i] = *reinterpret_cast<const uint4 *>(src + (tile_off(s, hz, hy, hx) * (kC * 2) + part * 16));
+        }
+    } else {
+        // loads from clamped, always valid addresses; the zero padding is
+        // applied when the chunk is written
+#pragma unroll
+        for (int i = 0; i < kStage; i++) {
+            const int c = min(int(threadIdx.x) + 256 * i, kHalo * 4 - 1);
+            const int hv = c >> 2, part = c & 3;
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            const int gz = z0 + hz - 1, gy = y0 + hy - 1, gx = x0 + hx - 1;
+            pad[i] = !inside(s, gz, gy, gx);
+            const int cz = min(max(gz, 0), s.D - 1), cy = min(max(gy, 0), s.H - 1);
+            const int cx = min(max(gx, 0), s.W - 1);
+            val[i] = reinterpret_cast<const uint4 *>(in + vox(s, b, cz, cy, cx) * kC)[part];
+        }
     }
 #pragma unroll
     for (int i = 0; i < kStage; i++) {
@@ -242,31 +307,48 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
 
     // weight fragments of tap + 1 are loaded while tap's products run
     constexpr int wn = NF * 16;  // rows (output channels) per tap of w
+    // byte offsets from the uniform weight base: one 32-bit add per tap
+    const char *wbase = reinterpret_cast<const char *>(w);
+    const unsigned wlane = unsigned(li * kC + lk) * 2;
     auto load_b = [&](int tap, bf16x8_t (&bf)[NF]) {
 #pragma unroll
         for (int f = 0; f < NF; f++)
-            bf[f] = *reinterpret_cast<const bf16x8_t *>(w + (size_t(tap) * wn + f * 16 + li) * kC + lk);
+            bf[f] = *reinterpret_cast<const bf16x8_t *>(wbase + (wlane + unsigned((tap * wn + f * 16) * kC * 2)));
     };
-    bf16x8_t bnext[NF];
-    load_b(0, bnext);
-    // unrolled: every halo and weight address is a lane base plus a constant
-    // (the LDS offsets go into the instructions), no per-tap index arithmetic
+    constexpr int kPD = 3;  // taps of weight fragments in flight
+    bf16x8_t bq[kPD][NF];
 #pragma unroll
-    for (int tap = 0; tap < kTaps; tap++) {
+    for (int p = 0; p < kPD; p++) load_b(p, bq[p]);
+    // A fragments (voxels x input channels) of one tap from the halo
+    const bf16_t *hb = halo + (wave * kHY * kHX + li) * kC + lk;  // this lane's halo base
+    auto load_a = [&](int tap, bf16x8_t (&af)[kTY]) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-        bf16x8_t bfrag[NF];
-#pragma unroll
-        for (int f = 0; f < NF; f++) bfrag[f] = bnext[f];
-        if (tap + 1 < kTaps) load_b(tap + 1, bnext);
 #pragma unroll
         for (int m = 0; m < kTY; m++) {
-            const int hz = wave + 1 + dz, hy = m + 1 + dy, hx = li + 1 + dx;
-            const bf16x8_t afrag =
-                *reinterpret_cast<const bf16x8_t *>(halo + ((hz * kHY + hy) * kHX + hx) * kC + lk);
+            af[m] = *reinterpret_cast<const bf16x8_t *>(hb + (((1 + dz) * kHY + m + 1 + dy) * kHX + 1 + dx) * kC);
+        }
+    };
+    // software pipeline: the halo reads of tap + 1 and the weight loads of
+    // tap + kPD are issued before tap's products, and the scheduler may not
+    // move them past those products (sched_barrier)
+    bf16x8_t a_cur[kTY], a_nxt[kTY];
+    load_a(0, a_cur);
+#pragma unroll
+    for (int tap = 0; tap < kTaps; tap++) {
+        bf16x8_t bfrag[NF];
+#pragma unroll
+        for (int f = 0; f < NF; f++) bfrag[f] = bq[tap % kPD][f];
+        if (tap + kPD < kTaps) load_b(tap + kPD, bq[tap % kPD]);
+        if (tap + 1 < kTaps) load_a(tap + 1, a_nxt);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = 0; m < kTY; m++)
 #pragma unroll
             for (int f = 0; f < NF; f++)
-                acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[f], acc[m][f], 0, 0, 0);
-        }
+                acc[m][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_cur[m], bfrag[f], acc[m][f], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int m = 0; m < kTY; m++) a_cur[m] = a_nxt[m];
     }
     __syncthreads();  // halo no longer needed: reuse it for the f32 output tile
     float *tile = reinterpret_cast<float *>(halo);  // [256 voxels][32] f32 = 32 KB
@@ -281,45 +363,58 @@ __global__ __launch_bounds__(256) void conv_mid_kernel(const bf16_t *__restrict_
                 tile[vl * kC + f * 16 + li] = acc[m][f][j];
             }
     __syncthreads();
+    const bool whole = tile_inside(s, o);
+    const long tile0 = vox(s, b, z0, y0, x0);  // the tile's first voxel
     if constexpr (MODE == 2) {
         // channel 0 of the tile, one output voxel per thread and pass
-        for (int vl = int(threadIdx.x); vl < kTZ * kTY * kTX; vl += 256) {
+        const float *xr = xres + tile0;
+        float *yo = yout + tile0;
+#pragma unroll
+        for (int k = 0; k < 1; k++) {
+            const int vl = int(threadIdx.x) + 256 * k;
             const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
-            const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
-            if (inside(s, gz, gy, gx)) {
-                const long gv = vox(s, b, gz, gy, gx);
-                yout[gv] = xres[gv] + (bias[0] + tile[vl * kC]);
+            if (whole || inside(s, z0 + zl, y0 + yl, x0 + xl)) {
+                const unsigned v = tile_off(s, zl, yl, xl);
+                yo[v] = xr[v] + (bias[0] + tile[vl * kC]);
             }
         }
         return;
     }
-    for (int c = threadIdx.x; c < kTZ * kTY * kTX * 4; c += 256) {
-        const int vl = c >> 2, part = c & 3;
+    // every pass of a thread handles the same 8 channels (256 is a multiple of 4)
+    const int part = threadIdx.x & 3;
+    float bv[8];
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) bv[e] = bias[part * 8 + e];
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(out + tile0 * kC) + part;
+    const uint4 *mk4 = MODE == 1 ? reinterpret_cast<const uint4 *>(mask + tile0 * kC) + part : nullptr;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int vl = (int(threadIdx.x) >> 2) + 64 * k;
         const int xl = vl % kTX, yl = (vl / kTX) % kTY, zl = vl / (kTX * kTY);
-        const int gz = z0 + zl, gy = y0 + yl, gx = x0 + xl;
-        if (!inside(s, gz, gy, gx)) continue;
-        const long gv = vox(s, b, gz, gy, gx);
+        if (!whole && !inside(s, z0 + zl, y0 + yl, x0 + xl)) continue;
+        const size_t v4 = size_t(tile_off(s, zl, yl, xl)) * 4;
         const float *src = tile + vl * kC + part * 8;
-        unsigned o[4];
-        if (MODE == 0) {
+        unsigned w2[4];
+        if constexpr (MODE == 0) {
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                const int ch = part * 8 + 2 * e;
-                const float a0 = fmaxf(src[2 * e] + bias[ch], 0.0f);
-                const float a1 = fmaxf(src[2 * e + 1] + bias[ch + 1], 0.0f);
-                o[e] = unsigned(f2bf(a0)) | (unsigned(f2bf(a1)) << 16);
+                const float a0 = fmaxf(src[2 * e] + bv[2 * e], 0.0f);
+                const float a1 = fmaxf(src[2 * e + 1] + bv[2 * e + 1], 0.0f);
+                w2[e] = pack_bf16x2(a0, a1);
             }
         } else {
-            const uint4 mk = reinterpret_cast<const uint4 *>(mask + gv * kC)[part];
+            const uint4 mk = mk4[v4];
             const unsigned mw[4] = {mk.x, mk.y, mk.z, mk.w};
 #pragma unroll
             for (int e = 0; e < 4; e++) {
                 const float a0 = bf2f(bf16_t(mw[e] & 0xffffu)) > 0.0f ? src[2 * e] : 0.0f;
                 const float a1 = bf2f(bf16_t(mw[e] >> 16)) > 0.0f ? src[2 * e + 1] : 0.0f;
-                o[e] = unsigned(f2bf(a0)) | (unsigned(f2bf(a1)) << 16);
+                w2[e] = pack_bf16x2(a0, a1);
             }
         }
-        reinterpret_cast<uint4 *>(out + gv * kC)[part] = make_uint4(o[0], o[1], o[2], o[3]);
+        dst[v4] = make_uint4(w2[0], w2[1], w2[2], w2[3]);
     }
 }
 
@@ -826,19 +921,32 @@ int refresh_operands(lct_cnn *n)
 
 unsigned blocks_for(long n) { return unsigned((n + 255) / 256); }
 
+// Grid of the conv kernels (see tile_origin); the shapes the kernels take
+int conv_grid(Shape s, dim3 *g)
+{
+    const long tx = (s.W + kTX - 1) / kTX, ty = (s.H + kTY - 1) / kTY;
+    const long tzb = long(s.B) * ((s.D + kTZ - 1) / kTZ);
+    if (long(s.H) * s.W > kMaxPlane)
+        return set_error(LCT_ERR_UNSUPPORTED, "cnn: a plane of more than 2^23 voxels");
+    if (ty > 65535 || tzb > 65535)
+        return set_error(LCT_ERR_UNSUPPORTED, "cnn: more than 65535 tile rows or batch x tile planes");
+    *g = dim3(unsigned(tx), unsigned(ty), unsigned(tzb));
+    return LCT_OK;
+}
+
 int forward(lct_cnn *n, const float *x, Shape s, float *y)
 {
     hipStream_t st = n->ctx->stream;
-    const long tiles = long(s.B) * ((s.D + kTZ - 1) / kTZ) * ((s.H + kTY - 1) / kTY) *
-                       ((s.W + kTX - 1) / kTX);
-    hipLaunchKernelGGL(conv_in1_kernel, dim3(unsigned(tiles)), dim3(256), 0, st, x, n->w1op,
+    dim3 grid;
+    LCT_CHECK(conv_grid(s, &grid));
+    hipLaunchKernelGGL(conv_in1_kernel, grid, dim3(256), 0, st, x, n->w1op,
                        n->w + kB1, n->act[1], s);
     for (int l = 0; l < 4; l++)
-        hipLaunchKernelGGL(conv_mid_kernel<0>, dim3(unsigned(tiles)), dim3(256), 0, st,
+        hipLaunchKernelGGL(conv_mid_kernel<0>, grid, dim3(256), 0, st,
                            n->act[l + 1], n->wb + size_t(l) * kTaps * kC * kC,
                            n->w + kMid + l * kMidStride + kTaps * kC * kC, nullptr, n->act[l + 2], s,
                            nullptr, nullptr);
-    hipLaunchKernelGGL(conv_mid_kernel<2>, dim3(unsigned(tiles)), dim3(256), 0, st, n->act[5],
+    hipLaunchKernelGGL(conv_mid_kernel<2>, grid, dim3(256), 0, st, n->act[5],
                        n->w6op, n->w + kB6, nullptr, nullptr, s, x, y);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
@@ -866,6 +974,8 @@ int backward(lct_cnn *n, const float *x, Shape s)
     const int vc = voxel_chunks(nv);
     const long tiles = conv_tiles(s);
     if (tiles > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "cnn: too many voxel tiles");
+    dim3 grid;
+    LCT_CHECK(conv_grid(s, &grid));
     const int tc = int(std::min<long>(kMaxChunks, tiles));
     // layer 6
     hipLaunchKernelGGL(wgrad_io_kernel<true>, dim3(tc), dim3(256), 0, st, n->act[5], n->dy,
@@ -885,7 +995,7 @@ int backward(lct_cnn *n, const float *x, Shape s)
         reduce_into(n, tc, long(kTaps) * kC * kC, gw);
         hipLaunchKernelGGL(bgrad_kernel, dim3(vc), dim3(256), 0, st, gk, nullptr, n->part, nv, vc);
         reduce_into(n, vc, kC, gw + kTaps * kC * kC);
-        hipLaunchKernelGGL(conv_mid_kernel<1>, dim3(unsigned(tiles)), dim3(256), 0, st, gk,
+        hipLaunchKernelGGL(conv_mid_kernel<1>, grid, dim3(256), 0, st, gk,
                            n->wt + size_t(l) * kTaps * kC * kC, nullptr, n->act[l + 1],
                            n->grad[cur ^ 1], s, nullptr, nullptr);
         cur ^= 1;
