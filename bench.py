@@ -111,6 +111,32 @@ def cpu_model():
     return "unknown"
 
 
+def cpu_pi_ori(w, fan_np, a, c, sample_slices):
+    """The FP64 oracle's PI-original stage set on block 0 (rebin, weight and
+    filter of its window, `sample_slices` backprojected) on all host threads:
+    the volume (NaN outside the sample) and its time scaled to a whole volume."""
+    from oracle import oracle as O
+    p = O.Prm()
+    pp = w.prm()
+    for f, _ in O.Prm._fields_:
+        setattr(p, f, getattr(pp, f))
+    pl = O.plan_helical(p, a, c)
+    kinds = dict(rebin_kind=2, weight_kind=1, bp_kind=1, window=w.window, blocks=(0, 1))
+    t0 = time.perf_counter()
+    O.reconstruct(pl, fan_np, a, c, iz_list=[], **kinds)
+    t_rf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vol = O.reconstruct(pl, fan_np, a, c, iz_list=list(sample_slices), **kinds)
+    t_slice = max(time.perf_counter() - t0 - t_rf, 0.0) / len(sample_slices)
+    per_volume = pl.n_blocks * (t_rf + 32 * t_slice)
+    threads = O.num_threads()
+    return ({"value": round(per_volume, 3), "unit": "s/volume", "cores": threads, "kind": "port",
+             "sample": f"FP64 oracle, {threads} threads, block 0 of {pl.n_blocks}: rebin+weight+"
+                       f"filter of its window ({t_rf:.2f} s) and {len(sample_slices)} of 32 slices "
+                       f"backprojected ({t_slice:.2f} s/slice), scaled to {pl.n_blocks} blocks x 32 "
+                       f"slices"}, vol)
+
+
 def cpu_baseline(w, fan_np, a, c, sample_slices=tuple(range(0, 32, 2))):
     """FP64 oracle on the host: block 0 (rebin + filter of its whole window, and
     `sample_slices` backprojected), scaled to a whole volume, on all host
@@ -301,7 +327,8 @@ def source_sha16(path=KERNEL_SOURCE):
         return hashlib.sha256(f.read()).hexdigest()[:16]
 
 
-def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_tap, n_vox):
+def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_tap, n_vox,
+                           rays=2, taps_from="LDS"):
     """Roofline of the backprojector launch against the unit that binds it,
     the VALU.  achieved = the VALU's busy cycles per launch, 4 x
     (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) quad-cycles from the
@@ -313,10 +340,11 @@ def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_
     bound), VALU lane-operations per update, LDS-active and bank-conflict
     fractions, measured HBM traffic (FETCH_SIZE / WRITE_SIZE, copy-calibrated)
     and its share of the 8 TB/s peak, and the gather-byte rate of the north
-    star's definition (2 rays x 4 taps x element size per update: the taps
-    come from LDS, not HBM)."""
+    star's definition (`rays` x 4 taps x element size per update; the folded
+    kernel's taps come from LDS, not HBM).  The counters match the source when
+    the committed hash is that of the file the profile names."""
     per_launch_updates = updates / launches
-    gather_bytes = (updates * 2 * 4 * bytes_per_tap + n_vox * 4) / launches
+    gather_bytes = (updates * rays * 4 * bytes_per_tap + n_vox * 4) / launches
     out = {"bound": "valu", "kernel": kernel}
     insts = prof.get("sq_insts_valu_per_launch")
     act, act2 = prof.get("sq_active_inst_valu_per_launch"), prof.get("sq_active_inst_valu2_per_launch")
@@ -346,12 +374,14 @@ def backprojector_roofline(prof, kernel, updates, launches, launch_s, bytes_per_
         "lds_bank_conflict": (round(prof["sq_lds_bank_conflict_per_launch"] / N_CU / cycles, 3)
                               if "sq_lds_bank_conflict_per_launch" in prof else None),
         "gather_GBps": round(gather_bytes / launch_s / 1e9, 1),
-        "gather_definition": f"2 rays x 4 bilinear taps x {bytes_per_tap} B per voxel-view update + "
-                             f"4 B per voxel written; {per_launch_updates:.4g} updates per launch; "
-                             f"served from LDS",
+        "gather_definition": f"{rays} ray{'s' if rays > 1 else ''} x 4 bilinear taps x {bytes_per_tap} B "
+                             f"per voxel-view update + 4 B per voxel written; "
+                             f"{per_launch_updates:.4g} updates per launch; served from {taps_from}",
         "launch_ms": round(launch_s * 1e3, 3),
         "counters": "profiles/" + prof.get("sq_source", "") if prof else None,
-        "counters_match_source": (prof.get("source_sha16") == source_sha16()) if prof else None,
+        "counters_match_source": (prof.get("source_sha16") == source_sha16(
+            os.path.join(ROOT, prof.get("source_file", os.path.relpath(KERNEL_SOURCE, ROOT)))))
+        if prof else None,
     })
     return out
 
@@ -417,6 +447,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip timing the learned correction's inference")
+    ap.add_argument("--no-pi-ori", action="store_true",
+                    help="skip the PI-original stage set (per-block FP32)")
     ap.add_argument("--variants", default="fast_f32,mixed_bf16",
                     help="comma-separated presets to time besides the headline path "
                          "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
@@ -540,6 +572,30 @@ def main():
             variants[name]["hu_vs_fp32_plain"] = hu_stats(variant_vols[name], plain_vol, np, scan)
         del vv
 
+    pi_ori, pi_vol = None, None
+    if not args.no_pi_ori:
+        # the reference's PI-original stage set: its rebin (Rebiner.cu:156-233),
+        # pre-weighting (PreWeighting.cu:8-44) and backprojector
+        # (Backprojector.cu:171-307), on the per-block FP32 pipeline
+        log("timing the PI-original stage set (per-block FP32)")
+        vp = torch.empty_like(vol)
+        r = time_path(ctx, pl, recon.preset("plain_pi_ori", window=w.window), a, c, fan, vp,
+                      max(1, min(args.steps, 3)), 1)
+        pi_vol = vp.cpu().numpy()
+        del vp
+        prof_pi = load_profile(f"bp_pi_ori_{w.name}")
+        pi_ori = {"s_per_volume": round(r["s_per_volume"], 6),
+                  "stage_ms": {k: round(x, 3) for k, x in r["stage_ms"].items()},
+                  "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"] * 1e-3)
+                                            / 1e9, 2),
+                  "updates_per_volume": r["updates"],
+                  "roofline": backprojector_roofline(
+                      prof_pi, prof_pi.get("kernel", "bp_pi_ori_literal_kernel"), r["updates"],
+                      r["bp_launches_per_volume"], r["bp_kernel_ms_per_launch"] * 1e-3, 4,
+                      pl.n_slices_recon * pl.ig.nx * pl.ig.ny, rays=1, taps_from="L2 / HBM"),
+                  "note": "one launch per 32-slice block; updates = voxels x half a turn of views"}
+        log(f"  {r['s_per_volume']*1e3:.2f} ms/volume  stages {r['stage_ms']}")
+
     learned = None
     if not args.no_learned:
         # inference cost of the learned correction on this volume (the trained
@@ -607,6 +663,8 @@ def main():
         out["cli_end_to_end"] = cli
     if variants:
         out["variants"] = variants
+    if pi_ori:
+        out["pi_original"] = pi_ori
     if learned:
         out["learned"] = learned
     if plain:
@@ -639,6 +697,14 @@ def main():
             s["rel_rmse"] = float(np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref))
             s["sample"] = f"block 0 slices {idx}, all voxels"
             out[f"hu_vs_fp64_oracle_{name}"] = s
+        if pi_vol is not None:
+            log("the FP64 oracle's PI-original stage set on the same sample")
+            ocb, opi = cpu_pi_ori(w, fan_np, a, c, idx)
+            s = hu_stats(pi_vol[idx], opi[idx], np, scan)
+            s["rel_rmse"] = float(np.linalg.norm(pi_vol[idx] - opi[idx]) / np.linalg.norm(opi[idx]))
+            s["sample"] = f"block 0 slices {idx}, all voxels"
+            out["pi_original"]["hu_vs_fp64_oracle"] = s
+            out["pi_original"]["cpu_baseline"] = ocb
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -213,7 +213,8 @@ typedef struct {
     float rebin_ms, filter_ms, backproject_ms, total_ms;
     int n_backproject_launches;
     long long backproject_updates; /* voxel-view updates (one per view in a
-                                      voxel's window) */
+                                      voxel's window; for LCT_BP_PI_ORI half
+                                      a turn of views per voxel) */
     float backproject_kernel_ms;   /* of backproject_ms, the backprojector
                                       launches alone (without the pass that
                                       adds the partial sums of split blocks) */

@@ -42,8 +42,14 @@ struct StageClock {
     }
 };
 
-long long count_updates(const lct_plan &plan)
+long long count_updates(const lct_plan &plan, int bp_kind)
 {
+    if (bp_kind == LCT_BP_PI_ORI) {
+        // half a turn of views per voxel (a_min + 1 .. a_min + npt / 2,
+        // Backprojector.cu:200-245) when its PI window lies inside the block's views
+        return (long long)(plan.ctg.n_proj_turn / 2) * LCT_BLOCK_SLICES * plan.ig.nx * plan.ig.ny *
+               plan.n_blocks;
+    }
     // one update per view in each voxel's window (a_max - a_min per slice)
     int amin[32], amax[32], amid[32];
     bp_windows(plan.ctg, plan.ig, amin, amax, amid);
@@ -148,7 +154,7 @@ static int reconstruct_range(lct_ctx *ctx, const lct_plan *plan, const lct_recon
     else
         rc = set_error(LCT_ERR_INVALID, std::string(who) + ": unknown pipeline");
     if (rc != LCT_OK) return rc;
-    ctx->timings.backproject_updates = count_updates(p) / p.n_blocks * (b1 - b0);
+    ctx->timings.backproject_updates = count_updates(p, o.bp_kind) / p.n_blocks * (b1 - b0);
     if (ctx->profiling) {
         LCT_CHECK_HIP(hipEventRecord(t1, ctx->stream));
         LCT_CHECK_HIP(hipEventSynchronize(t1));

@@ -140,6 +140,11 @@ def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
     if name == "resident_direct_f16":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16)
+    if name == "plain_pi_ori":
+        # the PI-original stage set (Rebiner.cu:156-233, PreWeighting.cu:8-44,
+        # Backprojector.cu:171-307) on the per-block FP32 pipeline
+        return Options(window=window, rebin_kind=_lib.REBIN_PI_ORI, weight_kind=_lib.WEIGHT_PI_ORI,
+                       bp_kind=_lib.BP_PI_ORI)
     if name == "fast_f32_toeplitz":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window,
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)

@@ -54,6 +54,14 @@ def test_bench_line_fields():
     assert d["hu_vs_fp64_oracle_mixed"]["rmse"] <= 1.0
     assert d["hu_vs_fp64_oracle_fp32_plain"]["rel_rmse"] <= 1e-4
     assert d["speedup_vs_fp32_plain"] > 1.0
+    # the PI-original stage set: timed, its roofline's measured rates, and
+    # against the oracle's PI-original reconstruction of the same sample
+    pi = d["pi_original"]
+    assert pi["s_per_volume"] > 0 and pi["backproject_gups"] > 0
+    assert pi["roofline"]["bound"] == "valu" and pi["roofline"]["launch_ms"] > 0
+    assert "L2" in pi["roofline"]["gather_definition"]
+    assert pi["hu_vs_fp64_oracle"]["rel_rmse"] <= 1e-4
+    assert pi["cpu_baseline"]["value"] > 0 and pi["cpu_baseline"]["kind"] == "port"
 
 
 def test_roofline_recomputes_from_the_counters():
