@@ -308,6 +308,12 @@ def cli_end_to_end(w, host_fan, head_vol, path):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def storage_of(path):
+    """Filtered-projection storage of a preset: f16, bf16 or f32."""
+    return {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16",
+            "mixed_f16_gather": "f16", "mixed_bf16_gather": "bf16"}.get(path, "f32")
+
+
 def load_profile(kernel_tag):
     """Per-launch counter figures of a kernel on a workload, committed under
     profiles/ (rocprofv3 runs); {} when there are none."""
@@ -572,29 +578,47 @@ def main():
             variants[name]["hu_vs_fp32_plain"] = hu_stats(variant_vols[name], plain_vol, np, scan)
         del vv
 
-    pi_ori, pi_vol = None, None
+    pi_ori, pi_vol, pi_plain_vol = None, None, None
     if not args.no_pi_ori:
         # the reference's PI-original stage set: its rebin (Rebiner.cu:156-233),
         # pre-weighting (PreWeighting.cu:8-44) and backprojector
-        # (Backprojector.cu:171-307), on the per-block FP32 pipeline
-        log("timing the PI-original stage set (per-block FP32)")
-        vp = torch.empty_like(vol)
-        r = time_path(ctx, pl, recon.preset("plain_pi_ori", window=w.window), a, c, fan, vp,
-                      max(1, min(args.steps, 3)), 1)
-        pi_vol = vp.cpu().numpy()
-        del vp
-        prof_pi = load_profile(f"bp_pi_ori_{w.name}")
-        pi_ori = {"s_per_volume": round(r["s_per_volume"], 6),
-                  "stage_ms": {k: round(x, 3) for k, x in r["stage_ms"].items()},
-                  "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"] * 1e-3)
-                                            / 1e9, 2),
-                  "updates_per_volume": r["updates"],
-                  "roofline": backprojector_roofline(
-                      prof_pi, prof_pi.get("kernel", "bp_pi_ori_literal_kernel"), r["updates"],
-                      r["bp_launches_per_volume"], r["bp_kernel_ms_per_launch"] * 1e-3, 4,
-                      pl.n_slices_recon * pl.ig.nx * pl.ig.ny, rays=1, taps_from="L2 / HBM"),
-                  "note": "one launch per 32-slice block; updates = voxels x half a turn of views"}
-        log(f"  {r['s_per_volume']*1e3:.2f} ms/volume  stages {r['stage_ms']}")
+        # (Backprojector.cu:171-307), on the headline path (view pairs folded:
+        # view j where its ray lies in the voxel's PI window, else view j + h)
+        # and on the reference's per-block FP32 loop
+        pi_ori = {}
+        runs = [("plain", "plain_pi_ori", "bp_pi_ori")]
+        if args.path != "plain" and "gather" not in args.path:  # no folded gather form
+            runs.insert(0, (args.path, f"{args.path}_pi_ori", "bp_fold_pi"))
+        for key, preset, tag in runs:
+            log(f"timing the PI-original stage set ({preset})")
+            vp = torch.empty_like(vol)
+            r = time_path(ctx, pl, recon.preset(preset, window=w.window), a, c, fan, vp,
+                          max(1, min(args.steps, 3)), 1)
+            if key == "plain":
+                pi_plain_vol = vp.cpu().numpy()
+            else:
+                pi_vol = vp.cpu().numpy()
+            del vp
+            st = "f32" if key in ("plain", "fast_f32") else storage_of(key)
+            prof_pi = load_profile(f"{tag}_{st}_{w.name}" if key != "plain" else f"{tag}_{w.name}")
+            pi_ori[key] = {
+                "s_per_volume": round(r["s_per_volume"], 6),
+                "stage_ms": {k: round(x, 3) for k, x in r["stage_ms"].items()},
+                "backproject_gups": round(r["updates"] / (r["stage_ms"]["backproject_ms"] * 1e-3)
+                                          / 1e9, 2),
+                "updates_per_volume": r["updates"],
+                "roofline": backprojector_roofline(
+                    prof_pi, prof_pi.get("kernel", "bp_pi_ori_literal_kernel" if key == "plain"
+                                         else "bp_fold_kernel"), r["updates"],
+                    r["bp_launches_per_volume"], r["bp_kernel_ms_per_launch"] * 1e-3,
+                    4 if st == "f32" else 2, pl.n_slices_recon * pl.ig.nx * pl.ig.ny, rays=1,
+                    taps_from="L2 / HBM" if key == "plain" else "LDS")}
+            log(f"  {r['s_per_volume']*1e3:.2f} ms/volume  stages {r['stage_ms']}")
+        if len(runs) == 2:
+            pi_ori["speedup_vs_plain"] = round(pi_ori["plain"]["s_per_volume"]
+                                               / pi_ori[args.path]["s_per_volume"], 3)
+        pi_ori["note"] = ("updates = voxels x half a turn of views; the plain loop launches once "
+                          "per 32-slice block, the folded backprojector once per volume")
 
     learned = None
     if not args.no_learned:
@@ -619,8 +643,7 @@ def main():
                            "them); trained accuracy in profiles/learned_pitch15.json"}
         torch.cuda.empty_cache()
 
-    storage = {"mixed_f16": "f16", "mixed_bf16": "bf16", "resident_direct_f16": "f16",
-               "mixed_f16_gather": "f16", "mixed_bf16_gather": "bf16"}.get(args.path, "f32")
+    storage = storage_of(args.path)
     bytes_per_tap = 2 if storage in ("f16", "bf16") else 4
     updates = head["updates"]
     bp_s = head["bp_kernel_ms_per_launch"] * 1e-3
@@ -700,10 +723,13 @@ def main():
         if pi_vol is not None:
             log("the FP64 oracle's PI-original stage set on the same sample")
             ocb, opi = cpu_pi_ori(w, fan_np, a, c, idx)
-            s = hu_stats(pi_vol[idx], opi[idx], np, scan)
-            s["rel_rmse"] = float(np.linalg.norm(pi_vol[idx] - opi[idx]) / np.linalg.norm(opi[idx]))
-            s["sample"] = f"block 0 slices {idx}, all voxels"
-            out["pi_original"]["hu_vs_fp64_oracle"] = s
+            for key, v in ((args.path, pi_vol), ("plain", pi_plain_vol)):
+                if v is None or key not in out["pi_original"]:
+                    continue
+                s = hu_stats(v[idx], opi[idx], np, scan)
+                s["rel_rmse"] = float(np.linalg.norm(v[idx] - opi[idx]) / np.linalg.norm(opi[idx]))
+                s["sample"] = f"block 0 slices {idx}, all voxels"
+                out["pi_original"][key]["hu_vs_fp64_oracle"] = s
             out["pi_original"]["cpu_baseline"] = ocb
     ctx.close()
     print(json.dumps(out), flush=True)

@@ -165,6 +165,9 @@ struct FoldBpArgs {
     int nx, ny, h;
     float wx, wy, wz, dx, dy, dz;
     float inv_dso, dso_ds, dso2, ws, wt, inv_orbit, z0, dzv, kc, ns_f, nt_f, scale;
+    float inv_ds;           // PI-original channels: p = s / ds + ws
+    int pi;                 // bp_windows_kernel: PI-original geometry and masks
+    unsigned *winb;         // PI-original: per (block, tile, pair) slices taking view j + h alone
 };
 
 template <typename T> struct Chunk {
@@ -250,9 +253,9 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // `shift` = (start mod 4) floats into its LDS region.  The two regions are
 // 4 Up floats apart (Up = units per view rounded up to 64), so each wave's
 // 64 units belong to one view and fill 1 KB of LDS in lane order.
-template <typename T, int NPRE, int LP, bool G = false, bool D = false>
+template <typename T, int NPRE, int LP, bool G = false, bool D = false, bool PI = false>
 __global__ __launch_bounds__(kBpThreads)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
+__attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !D) || PI ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
                                                       const unsigned *__restrict__ zmask,
                                                       const uint2 *__restrict__ win)
 {
@@ -293,7 +296,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
 
     // pairs of any slice's window: a_middle[0] - h <= j < a_middle[31]; this
     // workgroup takes [jb, je) of them (the tables are indexed from jlo)
-    const int jlo = st[2] - A.h, jhi = st[33];
+    // PI-original pairs: a_middle[0] - h < j <= a_middle[31] (one later)
+    const int jlo = st[2] - A.h + (PI ? 1 : 0), jhi = st[33] + (PI ? 1 : 0);
     const int jb = jlo + A.split[part];
     const int je = part == A.n_parts - 1 ? jhi : jlo + A.split[part + 1];
 
@@ -367,6 +371,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
 
     const float2 *csq = cs + qoff;
     const uint2 *wt = win + size_t(blk * tiles + t) * A.n_pairs;  // index j - jlo
+    const unsigned *wtb = PI ? A.winb + size_t(blk * tiles + t) * A.n_pairs : nullptr;
     if constexpr (D) {
         const unsigned w = wt[jb - jlo].x;
         dma(jb, int(w & 0xffffu), int(w >> 16));
@@ -383,6 +388,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
         const float c = cs1.x, s = cs1.y, c2 = cs2.x, s2v = cs2.y;
         const unsigned zm = zmask[j - jlo];
         const unsigned fm = wt[j - jlo].y;
+        const unsigned fm2 = PI ? wtb[j - jlo] : 0u;
         int s1 = 0, s2 = 0;
         const float *buf = lds;
         int view_off = lds_view;  // LDS offset of view j + h's window from view j's
@@ -405,7 +411,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
         const float phat = -xc * c - yc * s;
         const float u = phat * A.inv_dso;
         const float theta = A.asin_poly ? asin_poly(u) : asinf(u);
-        const float delta = theta * A.dso_ds;
+        // channel offset: theta dso / ds, or s / ds for the PI-original data
+        // (uniform in s, Backprojector.cu:215-216)
+        const float delta = PI ? phat * A.inv_ds : theta * A.dso_ds;
         const float p1 = fminf(fmaxf(A.ws + delta, -1.0f), A.ns_f);
         const float p2 = fminf(fmaxf(A.ws - delta, -1.0f), A.ns_f);
         const float fp1 = floorf(p1), fp2 = floorf(p2);
@@ -425,78 +433,135 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !D ? 6 : 1))) void bp_fold_
         const float tz = theta * A.inv_orbit;
         const float zr = A.z0 + (float(j) + tz) * A.dzv;
         const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
-        const float k1 = A.kc * __builtin_amdgcn_rcpf(lhat), k2 = A.kc * __builtin_amdgcn_rcpf(lhat2);
+        // PI-original rows (Backprojector.cu:207-216): t = root (zc - zr) / lhat +
+        // gamma P / 2 pi, row t 2 (nt - 1) / P + wt, with kc = z_dir 2 (nt - 1) / P
+        const float kr = PI ? A.kc * root : A.kc;
+        const float k1 = kr * __builtin_amdgcn_rcpf(lhat), k2 = kr * __builtin_amdgcn_rcpf(lhat2);
         // detector row offsets qh(z) = (zc(z) - zr) k1 as z K1 + O1 with zc(z) = (z - wz) dz:
         // the slice index is a literal, so no per-slice coordinate is held in a register
         const float K1 = k1 * A.dz, K2 = k2 * A.dz;
         // and the row coordinates in slot units (row + kRowPad) as r(z) = z K1 + R1,
         // R1 = O1 + row0, row0 = wt + kRowPad the slot of qh = 0
         const float row0 = A.wt + float(kRowPad);
-        const float R1 = fmaf(-A.wz, K1, fmaf(-zr, k1, row0));
-        const float R2 = fmaf(-A.wz, K2, fmaf(-zr2, k2, row0));
+        float R1, R2;
+        if constexpr (PI) {
+            const float gp = A.kc * tz * A.dzv;  // gamma P / 2 pi in rows (the ray through j + h: -gamma)
+            R1 = fmaf(-A.wz, K1, fmaf(-zr, k1, row0 + gp));
+            R2 = fmaf(-A.wz, K2, fmaf(-zr2, k2, row0 - gp));
+        } else {
+            R1 = fmaf(-A.wz, K1, fmaf(-zr, k1, row0));
+            R2 = fmaf(-A.wz, K2, fmaf(-zr2, k2, row0));
+        }
         // off the detector, a row coordinate is clamped to the guard slots
         // [kRowPad - 1, nt + kRowPad]: positive, so truncation is floor
         const float lo_r = float(kRowPad) - 1.0f, hi_r = A.nt_f + float(kRowPad);
+        if constexpr (PI) {
+            // Backprojector.cu:200-245: a voxel takes view j when that ray lies in
+            // its PI window (|t| <= P / 4: on the detector rows), else view j + h.
+            // The window table marks the slices where every column of the tile
+            // takes view j (fm, ray on the detector) or view j + h (fm2, ray j
+            // outside and ray j + h on the detector): one tap, no clamps.
+            auto tap = [&](const float *bl, const T *gl, float ap, float r) -> float {
+                const int i = int(r);
+                const float ar = __builtin_amdgcn_fractf(r);
+                float v00, v01, v10, v11;
+                if constexpr (G) {
+                    const T *q = gl + i;
+                    v00 = Cvt<T>::from(q[0]);
+                    v01 = Cvt<T>::from(q[1]);
+                    v10 = Cvt<T>::from(q[rp]);
+                    v11 = Cvt<T>::from(q[rp + 1]);
+                } else {
+                    const float *q = bl + i;
+                    v00 = q[0];
+                    v01 = q[1];
+                    v10 = q[lp];
+                    v11 = q[lp + 1];
+                }
+                const float lo = fmaf(ap, v10 - v00, v00), hi = fmaf(ap, v11 - v01, v01);
+                return fmaf(ar, hi - lo, lo);
+            };
 #pragma unroll
-        for (int z = 0; z < 32; z++) {
-            if (zm & (1u << z)) {  // pair inside this slice's window
-                float r1 = fmaf(float(z), K1, R1);
-                float r2 = fmaf(float(z), K2, R2);
-                // conjugate weights (w1, 1 - w1), w1 = qh2 / (qh2 - qh) with qh = r - row0
-                float w1 = (row0 - r2) * __builtin_amdgcn_rcpf(r1 - r2);
-                // keeps w1 computed on every lane, so the selects below stay
-                // selects instead of divergent branches (emits no instruction)
-                asm volatile("" : "+v"(w1));
-                // fm bit z (wave-uniform, from the window table): every column
-                // of the tile has both rays on the detector, so the row
-                // coordinates lie in the detector's slots and the detector
-                // tests, clamps and selects are skipped
-                if (!(fm & (1u << z))) {
-                    // keeps these branches: not speculated into selects (no instruction)
-                    asm volatile("");
-                    // a ray off the detector (|qh| > wt) takes weight 0, or 1/2
-                    // each when both are; when the compare masks show every
-                    // lane of the wave with both rays on the detector, the
-                    // clamps and selects are skipped
-                    const bool out1 = fabsf(r1 - row0) > A.wt, out2 = fabsf(r2 - row0) > A.wt;
-                    if (__builtin_amdgcn_ballot_w64(out1 || out2) != 0) {
+            for (int z = 0; z < 32; z++) {
+                const unsigned bit = 1u << z;
+                if (zm & bit) {
+                    if (fm & bit) {
+                        acc[z] += tap(b1, g1, ap1, fmaf(float(z), K1, R1));
+                    } else if (fm2 & bit) {
+                        acc[z] += tap(b2, g2, ap2, fmaf(float(z), K2, R2));
+                    } else {
+                        // per column: the view first, then its one tap (rows
+                        // clamped to the zero guard slots: the border)
                         asm volatile("");
-                        w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : w1);
-                        r1 = __builtin_amdgcn_fmed3f(r1, lo_r, hi_r);
-                        r2 = __builtin_amdgcn_fmed3f(r2, lo_r, hi_r);
+                        const float r1 = fmaf(float(z), K1, R1), r2 = fmaf(float(z), K2, R2);
+                        const bool in1 = fabsf(r1 - row0) <= A.wt;
+                        acc[z] += tap(in1 ? b1 : b2, in1 ? g1 : g2, in1 ? ap1 : ap2,
+                                      __builtin_amdgcn_fmed3f(in1 ? r1 : r2, lo_r, hi_r));
                     }
                 }
-                const int i1 = int(r1), i2 = int(r2);
-                const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
-                float a00, a01, a10, a11, c00, c01, c10, c11;
-                if constexpr (G) {
-                    const T *p1 = g1 + i1;
-                    const T *p2 = g2 + i2;
-                    a00 = Cvt<T>::from(p1[0]);
-                    a01 = Cvt<T>::from(p1[1]);
-                    a10 = Cvt<T>::from(p1[rp]);
-                    a11 = Cvt<T>::from(p1[rp + 1]);
-                    c00 = Cvt<T>::from(p2[0]);
-                    c01 = Cvt<T>::from(p2[1]);
-                    c10 = Cvt<T>::from(p2[rp]);
-                    c11 = Cvt<T>::from(p2[rp + 1]);
-                } else {
-                    const float *p1 = b1 + i1;
-                    const float *p2 = b2 + i2;
-                    a00 = p1[0];
-                    a01 = p1[1];
-                    a10 = p1[lp];
-                    a11 = p1[lp + 1];
-                    c00 = p2[0];
-                    c01 = p2[1];
-                    c10 = p2[lp];
-                    c11 = p2[lp + 1];
+            }
+        } else {
+    #pragma unroll
+            for (int z = 0; z < 32; z++) {
+                if (zm & (1u << z)) {  // pair inside this slice's window
+                    float r1 = fmaf(float(z), K1, R1);
+                    float r2 = fmaf(float(z), K2, R2);
+                    // conjugate weights (w1, 1 - w1), w1 = qh2 / (qh2 - qh) with qh = r - row0
+                    float w1 = (row0 - r2) * __builtin_amdgcn_rcpf(r1 - r2);
+                    // keeps w1 computed on every lane, so the selects below stay
+                    // selects instead of divergent branches (emits no instruction)
+                    asm volatile("" : "+v"(w1));
+                    // fm bit z (wave-uniform, from the window table): every column
+                    // of the tile has both rays on the detector, so the row
+                    // coordinates lie in the detector's slots and the detector
+                    // tests, clamps and selects are skipped
+                    if (!(fm & (1u << z))) {
+                        // keeps these branches: not speculated into selects (no instruction)
+                        asm volatile("");
+                        // a ray off the detector (|qh| > wt) takes weight 0, or 1/2
+                        // each when both are; when the compare masks show every
+                        // lane of the wave with both rays on the detector, the
+                        // clamps and selects are skipped
+                        const bool out1 = fabsf(r1 - row0) > A.wt, out2 = fabsf(r2 - row0) > A.wt;
+                        if (__builtin_amdgcn_ballot_w64(out1 || out2) != 0) {
+                            asm volatile("");
+                            w1 = out1 ? (out2 ? 0.5f : 0.0f) : (out2 ? 1.0f : w1);
+                            r1 = __builtin_amdgcn_fmed3f(r1, lo_r, hi_r);
+                            r2 = __builtin_amdgcn_fmed3f(r2, lo_r, hi_r);
+                        }
+                    }
+                    const int i1 = int(r1), i2 = int(r2);
+                    const float ar1 = __builtin_amdgcn_fractf(r1), ar2 = __builtin_amdgcn_fractf(r2);
+                    float a00, a01, a10, a11, c00, c01, c10, c11;
+                    if constexpr (G) {
+                        const T *p1 = g1 + i1;
+                        const T *p2 = g2 + i2;
+                        a00 = Cvt<T>::from(p1[0]);
+                        a01 = Cvt<T>::from(p1[1]);
+                        a10 = Cvt<T>::from(p1[rp]);
+                        a11 = Cvt<T>::from(p1[rp + 1]);
+                        c00 = Cvt<T>::from(p2[0]);
+                        c01 = Cvt<T>::from(p2[1]);
+                        c10 = Cvt<T>::from(p2[rp]);
+                        c11 = Cvt<T>::from(p2[rp + 1]);
+                    } else {
+                        const float *p1 = b1 + i1;
+                        const float *p2 = b2 + i2;
+                        a00 = p1[0];
+                        a01 = p1[1];
+                        a10 = p1[lp];
+                        a11 = p1[lp + 1];
+                        c00 = p2[0];
+                        c01 = p2[1];
+                        c10 = p2[lp];
+                        c11 = p2[lp + 1];
+                    }
+                    const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
+                    const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
+                    const float t1 = fmaf(ar1, ha - la, la);
+                    const float t2 = fmaf(ar2, hc - lc, lc);
+                    acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
                 }
-                const float la = fmaf(ap1, a10 - a00, a00), ha = fmaf(ap1, a11 - a01, a01);
-                const float lc = fmaf(ap2, c10 - c00, c00), hc = fmaf(ap2, c11 - c01, c01);
-                const float t1 = fmaf(ar1, ha - la, la);
-                const float t2 = fmaf(ar2, hc - lc, lc);
-                acc[z] = fmaf(w1, t1 - t2, acc[z] + t2);
             }
         }
         if constexpr (D) {
@@ -563,11 +628,31 @@ __device__ __forceinline__ void pair_row_lines(const FoldBpArgs &A, float x, flo
     const float tz = theta * A.inv_orbit;
     const float zr = A.z0 + (float(j) + tz) * A.dzv;
     const float zr2 = A.z0 + (float(j + A.h) - tz) * A.dzv;
-    const float k1 = A.kc * __builtin_amdgcn_rcpf(lhat), k2 = A.kc * __builtin_amdgcn_rcpf(lhat2);
+    // PI-original rows: kc = z_dir 2 (nt - 1) / P, t = root (zc - zr) / lhat +- gamma P / 2 pi
+    const float kr = A.pi ? A.kc * root : A.kc;
+    const float gp = A.pi ? A.kc * tz * A.dzv : 0.0f;
+    const float k1 = kr * __builtin_amdgcn_rcpf(lhat), k2 = kr * __builtin_amdgcn_rcpf(lhat2);
     K1 = k1 * A.dz;
     K2 = k2 * A.dz;
-    O1 = -A.wz * K1 - zr * k1;
-    O2 = -A.wz * K2 - zr2 * k2;
+    O1 = -A.wz * K1 - zr * k1 + gp;
+    O2 = -A.wz * K2 - zr2 * k2 - gp;
+}
+
+// Narrows [zlo, zhi] to the slices z with lo <= z K + O <= hi (K != 0).
+__device__ __forceinline__ void clip_line(float K, float O, float lo, float hi, float &zlo, float &zhi)
+{
+    const float r = __builtin_amdgcn_rcpf(K);
+    const float za = (lo - O) * r, zb = (hi - O) * r;
+    zlo = fmaxf(zlo, fminf(za, zb));
+    zhi = fminf(zhi, fmaxf(za, zb));
+}
+
+// The slices 0..31 inside [zlo, zhi] (one within 1e-3 of an end is left out).
+__device__ __forceinline__ unsigned slice_bits(float zlo, float zhi)
+{
+    const int z0 = int(ceilf(fminf(fmaxf(zlo + 1e-3f, 0.0f), 32.0f)));
+    const int z1 = int(floorf(fmaxf(fminf(zhi - 1e-3f, 31.0f), -1.0f)));
+    return z1 < z0 ? 0u : (z1 - z0 == 31 ? 0xffffffffu : ((1u << (z1 - z0 + 1)) - 1u) << z0);
 }
 
 // Per (block, tile, pair):
@@ -597,10 +682,10 @@ __global__ __launch_bounds__(256) void bp_windows_kernel(FoldBpArgs A, int n_blo
     const float xcc = (float(tx0) + 0.5f * (kTile - 1) - A.wx) * A.dx;
     const float ycc = (float(ty0) + 0.5f * (kTileY - 1) - A.wy) * A.dy;
     const int *st = A.slab_tab + blk * 34;
-    const int j = st[2] - A.h + jj;
+    const int j = st[2] - A.h + (A.pi ? 1 : 0) + jj;
     const float2 c = cs[st[0] + j];
     const float ph = -xcc * c.x - ycc * c.y;
-    const float c1 = A.ws + asinf(ph * A.inv_dso) * A.dso_ds;
+    const float c1 = A.ws + (A.pi ? ph * A.inv_ds : asinf(ph * A.inv_dso) * A.dso_ds);
     const int s1 = window_start(c1, A.H, A.cp, A.W);
     const int s2 = window_start(2.0f * A.ws - c1, A.H, A.cp, A.W);
 
@@ -630,6 +715,30 @@ __global__ __launch_bounds__(256) void bp_windows_kernel(FoldBpArgs A, int n_blo
         e = fmaxf(e, 2.0f * (fmaxf(dx0, dx1) + fmaxf(dy0, dy1)));
     }
     const float L = A.wt - 0.25f - e;
+    if (A.pi) {
+        // PI-original: fast = every column's ray j inside its PI window (|qh| <= wt,
+        // here <= L: on the detector); winb = every column's ray j outside it on
+        // one side (|qh| >= wt + 1/4 + e) and ray j + h on the detector (|qh| <= L)
+        const float U = A.wt + 0.25f + e;
+        float in_lo = -1e30f, in_hi = 1e30f, up_lo = -1e30f, up_hi = 1e30f, dn_lo = -1e30f,
+              dn_hi = 1e30f, r2_lo = -1e30f, r2_hi = 1e30f;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                clip_line(K[a][b][0], O[a][b][0], -L, L, in_lo, in_hi);
+                clip_line(K[a][b][0], O[a][b][0], U, 1e30f, up_lo, up_hi);
+                clip_line(K[a][b][0], O[a][b][0], -1e30f, -U, dn_lo, dn_hi);
+                clip_line(K[a][b][1], O[a][b][1], -L, L, r2_lo, r2_hi);
+            }
+        const unsigned fast = L > 0.0f ? slice_bits(in_lo, in_hi) : 0u;
+        const unsigned other = L > 0.0f ? (slice_bits(up_lo, up_hi) | slice_bits(dn_lo, dn_hi)) &
+                                              slice_bits(r2_lo, r2_hi)
+                                        : 0u;
+        win[i] = make_uint2(unsigned(s1) | (unsigned(s2) << 16), fast);
+        A.winb[i] = other & ~fast;
+        return;
+    }
     float zlo = -1e30f, zhi = 1e30f;
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -673,7 +782,7 @@ static int launch_bp_fold_gather(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, cons
     return LCT_OK;
 }
 
-template <typename T>
+template <typename T, bool PI>
 static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks * a.n_parts;
@@ -693,9 +802,10 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTab
     using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const uint2 *);
     static const Kern table[4][8] = {
 #define LCT_ROW(LP)                                                                              \
-    {bp_fold_kernel<T, 1, LP>, bp_fold_kernel<T, 2, LP>, bp_fold_kernel<T, 3, LP>,               \
-     bp_fold_kernel<T, 4, LP>, bp_fold_kernel<T, 5, LP>, bp_fold_kernel<T, 6, LP>,               \
-     bp_fold_kernel<T, 7, LP>, bp_fold_kernel<T, 8, LP>}
+    {bp_fold_kernel<T, 1, LP, false, false, PI>, bp_fold_kernel<T, 2, LP, false, false, PI>,     \
+     bp_fold_kernel<T, 3, LP, false, false, PI>, bp_fold_kernel<T, 4, LP, false, false, PI>,     \
+     bp_fold_kernel<T, 5, LP, false, false, PI>, bp_fold_kernel<T, 6, LP, false, false, PI>,     \
+     bp_fold_kernel<T, 7, LP, false, false, PI>, bp_fold_kernel<T, 8, LP, false, false, PI>}
         LCT_ROW(0), LCT_ROW(25), LCT_ROW(73), LCT_ROW(105)
 #undef LCT_ROW
     };
@@ -711,6 +821,7 @@ static int launch_bp_fold(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTab
 }
 
 // f32 storage: windows copied by LDS-DMA, Q with the odd pitch rp = lp.
+template <bool PI>
 static int launch_bp_fold_dma(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const BpTables &tb)
 {
     a.n_wg = a.n_tiles_x * a.n_tiles_y * n_blocks * a.n_parts;
@@ -722,10 +833,10 @@ static int launch_bp_fold_dma(lct_ctx *ctx, FoldBpArgs &a, int n_blocks, const B
     if (shm > 160 * 1024)
         return set_error(LCT_ERR_UNSUPPORTED, "backprojector: channel window exceeds LDS");
     using Kern = void (*)(FoldBpArgs, const float2 *, const unsigned *, const uint2 *);
-    const Kern k = a.rp == 25    ? bp_fold_kernel<float, 1, 25, false, true>
-                   : a.rp == 73  ? bp_fold_kernel<float, 1, 73, false, true>
-                   : a.rp == 105 ? bp_fold_kernel<float, 1, 105, false, true>
-                                 : bp_fold_kernel<float, 1, 0, false, true>;
+    const Kern k = a.rp == 25    ? bp_fold_kernel<float, 1, 25, false, true, PI>
+                   : a.rp == 73  ? bp_fold_kernel<float, 1, 73, false, true, PI>
+                   : a.rp == 105 ? bp_fold_kernel<float, 1, 105, false, true, PI>
+                                 : bp_fold_kernel<float, 1, 0, false, true, PI>;
     if (shm > 64 * 1024)
         LCT_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(k),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(shm)));
@@ -804,7 +915,11 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     // Can the pair-folded backprojector be used?
     int amin[32], amax[32], amid[32];
     bp_windows(g, ig, amin, amax, amid);
-    bool fold = (g.n_proj_turn % 2 == 0) && o.bp_kind == LCT_BP_HELICAL && (g.nt % 8 == 0);
+    // the PI-original backprojector folds the same pairs (one view of each
+    // counts, Backprojector.cu:200-245) but has no cache-gather form
+    const bool pi = o.bp_kind == LCT_BP_PI_ORI;
+    bool fold = (g.n_proj_turn % 2 == 0) && (g.nt % 8 == 0) &&
+                (o.bp_kind == LCT_BP_HELICAL || (pi && o.interp != LCT_INTERP_GATHER));
     for (int z = 0; z < 32 && fold; z++)
         fold = (amid[z] - h >= 0) && (amid[z] + h < g.n_angle) && (z == 0 || amid[z] >= amid[z - 1]);
     for (long v = R.g0; v + h < R.g0 + R.nviews && fold; v++) {
@@ -944,12 +1059,16 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         tab[34 * k + 1] = k * LCT_BLOCK_SLICES;
         for (int z = 0; z < 32; z++) tab[34 * k + 2 + z] = amid[z];
     }
+    // helical: pairs a_middle - h <= j < a_middle of each slice (Backprojector.cu:38-47);
+    // PI-original: a_middle - h < j <= a_middle, the views a_middle - h + 1 ..
+    // a_middle + h among which the PI window of every voxel lies (its walk,
+    // Backprojector.cu:200-245, starts at a_min > a_middle - h)
     std::vector<unsigned> zmask(size_t(amid[31] - amid[0] + h));
     for (size_t jj = 0; jj < zmask.size(); jj++) {
-        const int j = amid[0] - h + int(jj);
+        const int j = amid[0] - h + (pi ? 1 : 0) + int(jj);
         unsigned m = 0;
         for (int z = 0; z < 32; z++)
-            if (amid[z] - h <= j && j < amid[z]) m |= 1u << z;
+            if (pi ? (amid[z] - h < j && j <= amid[z]) : (amid[z] - h <= j && j < amid[z])) m |= 1u << z;
         zmask[jj] = m;
     }
     const size_t tab_bytes = cs_host.size() * 4 + tab.size() * 4 + zmask.size() * 4;
@@ -961,7 +1080,8 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     const int n_tiles = ((ig.nx + kTile - 1) / kTile) * ((ig.ny + kTileY - 1) / kTileY);
     const long n_win = long(nb) * n_tiles * n_pairs;
     void *d_win = nullptr;
-    LCT_CHECK(scratch(ctx, kScratchMisc0, size_t(n_win) * 8, &d_win));
+    // window table (8 B per entry), then the PI-original view-(j + h) slices (4 B)
+    LCT_CHECK(scratch(ctx, kScratchMisc0, size_t(n_win) * (pi ? 12 : 8), &d_win));
     const BpTables tb{reinterpret_cast<const float2 *>(d_cs), d_zm, static_cast<const uint2 *>(d_win)};
 
     FoldBpArgs a{};
@@ -994,10 +1114,18 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     a.inv_orbit = float(1.0 / double(g.orbit));
     a.z0 = g.source_z0;
     a.dzv = float(double(g.pitch) / double(g.n_proj_turn));
-    a.kc = float(double(g.dso) * double(g.z_dir) / double(g.dt));
+    // rows per mm along z: dso z_dir / dt for the helical data, z_dir 2 (nt - 1) / P
+    // for the PI-original rebin (rows P / 2 (nt - 1) apart, Backprojector.cu:216)
+    a.kc = pi ? float(double(g.z_dir) * 2.0 * double(g.nt - 1) / double(g.pitch))
+              : float(double(g.dso) * double(g.z_dir) / double(g.dt));
+    a.inv_ds = float(1.0 / double(g.ds));
+    a.pi = pi ? 1 : 0;
+    a.winb = pi ? reinterpret_cast<unsigned *>(static_cast<char *>(d_win) + size_t(n_win) * 8) : nullptr;
     a.ns_f = float(g.ns);
     a.nt_f = float(g.nt);
-    a.scale = float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
+    // helical: 2 pi / (a_max - a_min) over 2h views, each pair counted once here
+    // (twice there); PI-original: pi / (npt / 2) (Backprojector.cu:247)
+    a.scale = pi ? float(LCT_PI_D / double(h)) : float(2.0 * 2.0 * LCT_PI_D / double(2 * h));
     // Each (block, tile) can be backprojected by kParts workgroups, one per
     // range of the block's pairs (ranges of about equal work: a pair costs
     // about 4 slices' worth of set-up and staging plus its active slices).
@@ -1042,6 +1170,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         b.slab_tab = d_st + 34 * k0;
         BpTables tk = tb;  // the launch numbers its blocks from 0
         tk.win += size_t(k0) * n_tiles * n_pairs;
+        if (b.winb) b.winb += size_t(k0) * n_tiles * n_pairs;
         LCT_CHECK(tic());
         if (o.interp == LCT_INTERP_GATHER) {
             if (o.storage == LCT_F32)
@@ -1051,11 +1180,14 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
             else
                 LCT_CHECK(launch_bp_fold_gather<__hip_bfloat16>(ctx, b, k1 - k0, tk));
         } else if (o.storage == LCT_F32)
-            LCT_CHECK(launch_bp_fold_dma(ctx, b, k1 - k0, tk));
+            LCT_CHECK(pi ? launch_bp_fold_dma<true>(ctx, b, k1 - k0, tk)
+                         : launch_bp_fold_dma<false>(ctx, b, k1 - k0, tk));
         else if (o.storage == LCT_F16)
-            LCT_CHECK(launch_bp_fold<__half>(ctx, b, k1 - k0, tk));
+            LCT_CHECK((pi ? launch_bp_fold<__half, true>(ctx, b, k1 - k0, tk)
+                          : launch_bp_fold<__half, false>(ctx, b, k1 - k0, tk)));
         else
-            LCT_CHECK(launch_bp_fold<__hip_bfloat16>(ctx, b, k1 - k0, tk));
+            LCT_CHECK((pi ? launch_bp_fold<__hip_bfloat16, true>(ctx, b, k1 - k0, tk)
+                          : launch_bp_fold<__hip_bfloat16, false>(ctx, b, k1 - k0, tk)));
         // the backprojector alone, for its launch time (the sum pass follows)
         if (ctx->profiling) LCT_CHECK_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
         if (a.n_parts > 1) {

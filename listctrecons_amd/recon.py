@@ -140,14 +140,16 @@ def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
     if name == "resident_direct_f16":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16)
-    if name == "plain_pi_ori":
-        # the PI-original stage set (Rebiner.cu:156-233, PreWeighting.cu:8-44,
-        # Backprojector.cu:171-307) on the per-block FP32 pipeline
-        return Options(window=window, rebin_kind=_lib.REBIN_PI_ORI, weight_kind=_lib.WEIGHT_PI_ORI,
-                       bp_kind=_lib.BP_PI_ORI)
     if name == "fast_f32_toeplitz":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window,
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
+    if name.endswith("_pi_ori"):
+        # the PI-original stage set (Rebiner.cu:156-233, PreWeighting.cu:8-44,
+        # Backprojector.cu:171-307) on the named path: "plain_pi_ori" is the
+        # reference's per-block loop, the resident ones fold its view pairs
+        o = preset(name[: -len("_pi_ori")], window)
+        o.rebin_kind, o.weight_kind, o.bp_kind = _lib.REBIN_PI_ORI, _lib.WEIGHT_PI_ORI, _lib.BP_PI_ORI
+        return o
     if name.endswith("_gather"):
         o = preset(name[: -len("_gather")], window)
         o.interp = _lib.INTERP_GATHER

@@ -57,10 +57,13 @@ def test_bench_line_fields():
     # the PI-original stage set: timed, its roofline's measured rates, and
     # against the oracle's PI-original reconstruction of the same sample
     pi = d["pi_original"]
-    assert pi["s_per_volume"] > 0 and pi["backproject_gups"] > 0
-    assert pi["roofline"]["bound"] == "valu" and pi["roofline"]["launch_ms"] > 0
-    assert "L2" in pi["roofline"]["gather_definition"]
-    assert pi["hu_vs_fp64_oracle"]["rel_rmse"] <= 1e-4
+    for key in ("mixed_f16", "plain"):
+        assert pi[key]["s_per_volume"] > 0 and pi[key]["backproject_gups"] > 0
+        assert pi[key]["roofline"]["bound"] == "valu" and pi[key]["roofline"]["launch_ms"] > 0
+    assert "L2" in pi["plain"]["roofline"]["gather_definition"]
+    assert pi["plain"]["hu_vs_fp64_oracle"]["rel_rmse"] <= 1e-4
+    assert pi["mixed_f16"]["hu_vs_fp64_oracle"]["rmse"] <= 1.0
+    assert pi["speedup_vs_plain"] > 1.0
     assert pi["cpu_baseline"]["value"] > 0 and pi["cpu_baseline"]["kind"] == "port"
 
 

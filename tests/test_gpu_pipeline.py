@@ -64,6 +64,34 @@ def test_reduced_storage_within_1hu(ctx, cfg1, preset):
     assert rmse_hu <= 1.0
 
 
+@pytest.fixture(scope="module")
+def cfg1_pi():
+    w = scan.workload("helical_16x368_256x32")
+    pl, a, c = oracle_plan(w)
+    fan = cpu_fan(w)
+    ref = O.reconstruct(pl, fan, a, c, rebin_kind=2, weight_kind=1, bp_kind=1, window=w.window)
+    return w, fan, ref
+
+
+@pytest.mark.parametrize("preset,hu", [("plain_pi_ori", None), ("fast_f32_pi_ori", None),
+                                       ("mixed_f16_pi_ori", 1.0), ("mixed_bf16_pi_ori", 1.0)])
+def test_pi_original_paths_match_oracle(ctx, cfg1_pi, preset, hu):
+    """The PI-original stage set against the oracle's: the reference's
+    per-block loop and the folded resident backprojector (view j of a pair
+    where its ray lies in the voxel's PI window, else view j + h) within
+    1e-4 relative RMSE in FP32, within 1 HU with 16-bit storage."""
+    w, fan, ref = cfg1_pi
+    print(f"{preset} on {w.name}", flush=True)
+    got = gpu_recon(ctx, w, fan, preset)
+    err = rel_rmse(got, ref)
+    rmse_hu, max_hu = hu_errors(got, ref)
+    print(f"  rel rmse {err:.2e}  rmse {rmse_hu:.4f} HU  max {max_hu:.3f} HU", flush=True)
+    if hu is None:
+        assert err <= 1e-4
+    else:
+        assert rmse_hu <= hu
+
+
 VARIANTS = [
     ("tiny", {}, {}),
     ("tiny", {"z_dir": -1}, {}),
@@ -71,6 +99,9 @@ VARIANTS = [
     ("tiny", {"pitch": 1.7, "readings": 1400}, {}),
     ("tiny", {}, {"rebin_kind": 0}),
     ("tiny", {}, {"rebin_kind": 2, "weight_kind": 1, "bp_kind": 1}),
+    ("tiny", {"z_dir": -1}, {"rebin_kind": 2, "weight_kind": 1, "bp_kind": 1}),
+    ("tiny", {"pitch": 1.7, "readings": 1400}, {"rebin_kind": 2, "weight_kind": 1, "bp_kind": 1}),
+    ("tiny", {"dx": 14.0}, {"rebin_kind": 2, "weight_kind": 1, "bp_kind": 1}),
     ("tiny", {"n_proj_turn": 97, "readings": 1200}, {}),  # odd npt: no pair folding
     # volume corners at |p_hat| / dso ~ 0.6 (the backprojector's asin
     # polynomial) and ~ 0.73 (beyond its range: the library asin)
