@@ -204,7 +204,12 @@ typedef struct {
     int filter_algo; /* LCT_FILTER_* (RESIDENT only; BLOCKS uses DIRECT_F32) */
     int storage;     /* LCT_F32 / LCT_F16 / LCT_BF16: filtered-projection
                         storage of the RESIDENT backprojector */
-    int bp_kind;     /* LCT_BP_* */
+    int bp_kind;     /* LCT_BP_*; on RESIDENT both kinds fold the view pairs
+                        (j, j + npt/2): HELICAL with the conjugate weights,
+                        PI_ORI taking view j where its ray lies in the voxel's
+                        PI window and view j + npt/2 otherwise (no INTERP_GATHER
+                        form: PI_ORI with INTERP_GATHER backprojects block by
+                        block, F32 only) */
     int interp;      /* LCT_INTERP_* (RESIDENT only) */
 } lct_recon_opts;
 
