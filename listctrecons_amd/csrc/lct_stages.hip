@@ -75,48 +75,73 @@ __global__ __launch_bounds__(256) void rebin_views_lerp_kernel(
 // 0.99 with 2, 0.82 with 4, 0.71 with 8, 0.72 with 16).
 constexpr int kRebinViews = 8;
 
-// CUDA/Rebiner.cu:156-179 (PI_ORI): trilinear sample at (pI, qI, bI).
+// CUDA/Rebiner.cu:156-179 (PI_ORI): trilinear sample at (pI, qI, bI), with
+// pI and the view fraction depending on the channel only and qI on (row,
+// channel). A thread makes V consecutive output views of one channel, row by
+// row: the V + 1 fan views they read are sampled bilinearly at (qI, pI) once
+// each and the outputs lerp neighbouring samples (as the fan-to-parallel
+// rebin above). Taps outside the views, rows or channels read zero (the
+// texture's border mode).
+template <int V>
 __global__ __launch_bounds__(256) void rebin_pi_ori_kernel(
     const float *__restrict__ fan, long fan_views, float *__restrict__ par, long base, int ns,
     int nt, const int *__restrict__ pi0, const float *__restrict__ pfr,
     const int *__restrict__ vk0, const float *__restrict__ vfr, const float *__restrict__ qc1,
-    const float *__restrict__ qc0, float wt, const float *__restrict__ wtab)
+    const float *__restrict__ qc0, float wt, const float *__restrict__ wtab, long n_out)
 {
-    const int is = blockIdx.x * blockDim.x + threadIdx.x;
-    const int it = blockIdx.y;
-    const long ia = blockIdx.z;
-    if (is >= ns) return;
+    const long n_wg = long(gridDim.x);
+    const long wg = long(blockIdx.x & 7u) * (n_wg >> 3) + long(blockIdx.x >> 3);
+    const long n_cx = (ns + 255) / 256;
+    const long grp = wg / n_cx;
+    const int is = int(wg - grp * n_cx) * 256 + int(threadIdx.x);
+    const long ia0 = grp * V;
+    if (ia0 >= n_out || is >= ns) return;
+    const int nv = int(n_out - ia0 < V ? n_out - ia0 : V);
     const int x0 = pi0[is];
     const float ax = pfr[is];
-    const float qI = fmaf(float(it) - wt, qc1[is], qc0[is]);
-    const float fy = floorf(qI);
-    const int y0 = int(fy);
-    const float ay = qI - fy;
-    const long z0 = base + ia + vk0[is];
+    const long z0 = base + ia0 + vk0[is];
     const float az = vfr[is];
+    const float c1 = qc1[is], c0 = qc0[is];
     const size_t view = size_t(nt) * ns;
-    float acc = 0.0f;
+    const bool okx0 = x0 >= 0 && x0 < ns, okx1 = x0 + 1 >= 0 && x0 + 1 < ns;
+    const long xo = okx0 ? x0 : (okx1 ? x0 + 1 : 0);  // an in-range channel to address from
+    bool okz[V + 1];
+    size_t offz[V + 1];
 #pragma unroll
-    for (int dz = 0; dz < 2; dz++) {
-        const long zz = z0 + dz;
-        if (zz < 0 || zz >= fan_views) continue;
-        const float wz = dz ? az : 1.0f - az;
+    for (int j = 0; j <= V; j++) {
+        okz[j] = z0 + j >= 0 && z0 + j < fan_views;
+        offz[j] = okz[j] ? size_t(z0 + j) * view + size_t(xo) : 0;
+    }
+    const long d0 = long(x0) - xo, d1 = d0 + 1;  // channel offsets of the two taps from xo
+    float *o = par + size_t(ia0) * view + is;
+    for (int it = 0; it < nt; it++) {
+        const float qI = fmaf(float(it) - wt, c1, c0);
+        const float fy = floorf(qI);
+        const int y0 = int(fy);
+        const float ay = qI - fy;
+        const bool oky0 = y0 >= 0 && y0 < nt, oky1 = y0 + 1 >= 0 && y0 + 1 < nt;
+        const long r0 = long(y0) * ns, r1 = r0 + ns;
+        float smp[V + 1];
 #pragma unroll
-        for (int dy = 0; dy < 2; dy++) {
-            const int yy = y0 + dy;
-            if (yy < 0 || yy >= nt) continue;
-            const float wy = dy ? ay : 1.0f - ay;
+        for (int j = 0; j <= V; j++) {
+            const float *f = fan + offz[j];
+            const float v00 = okz[j] && oky0 && okx0 ? f[r0 + d0] : 0.0f;
+            const float v01 = okz[j] && oky0 && okx1 ? f[r0 + d1] : 0.0f;
+            const float v10 = okz[j] && oky1 && okx0 ? f[r1 + d0] : 0.0f;
+            const float v11 = okz[j] && oky1 && okx1 ? f[r1 + d1] : 0.0f;
+            const float lo = fmaf(ax, v01 - v00, v00), hi = fmaf(ax, v11 - v10, v10);
+            smp[j] = fmaf(ay, hi - lo, lo);
+        }
+        const size_t r = size_t(it) * ns;
+        const float w = wtab ? wtab[r + is] : 1.0f;
 #pragma unroll
-            for (int dx = 0; dx < 2; dx++) {
-                const int xx = x0 + dx;
-                if (xx < 0 || xx >= ns) continue;
-                const float wx = dx ? ax : 1.0f - ax;
-                acc += wz * wy * wx * fan[size_t(zz) * view + size_t(yy) * ns + xx];
-            }
+        for (int j = 0; j < V; j++) {
+            if (j >= nv) break;
+            float v = fmaf(az, smp[j + 1] - smp[j], smp[j]);
+            if (wtab) v *= w;
+            o[size_t(j) * view + r] = v;
         }
     }
-    if (wtab) acc *= wtab[size_t(it) * ns + is];
-    par[size_t(ia) * view + size_t(it) * ns + is] = acc;
 }
 
 // CUDA/PreWeighting.cu:8-29: par[ia][it][is] *= w[it][is].
@@ -196,10 +221,13 @@ int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *gp, cons
     const float *fan = d_fan + size_t(fan_view0) * ns * nt;
     if (n_out <= 0) return LCT_OK;
     if (kind == LCT_REBIN_PI_ORI) {
-        dim3 grid((ns + 255) / 256, nt, unsigned(n_out));
-        hipLaunchKernelGGL(rebin_pi_ori_kernel, grid, dim3(256), 0, ctx->stream, fan, fan_views,
-                           d_par, out_view0, ns, nt, d_i, d_f, d_i + ns, d_f + ns, d_f + 2 * ns,
-                           d_f + 3 * ns, g.wt, d_w);
+        constexpr int V = kRebinViews;
+        const long n_grp = (n_out + V - 1) / V;
+        const long n_wg = ((long((ns + 255) / 256) * n_grp) + 7) & ~7L;
+        if (n_wg > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "rebin: too many views");
+        hipLaunchKernelGGL(rebin_pi_ori_kernel<V>, dim3(unsigned(n_wg)), dim3(256), 0, ctx->stream,
+                           fan, fan_views, d_par, out_view0, ns, nt, d_i, d_f, d_i + ns, d_f + ns,
+                           d_f + 2 * ns, d_f + 3 * ns, g.wt, d_w, long(n_out));
     } else {
         constexpr int V = kRebinViews;
         const long n_grp = (n_out + V - 1) / V;
