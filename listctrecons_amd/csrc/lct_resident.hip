@@ -21,6 +21,15 @@
 // whose window contains the pair.  Geometries where the folding does not
 // hold (odd npt, clipped windows, irregular angles) use the per-voxel
 // kernel of lct_stages.hip on the same resident arrays.
+//
+// The PI-original backprojector (CUDA/Backprojector.cu:171-250) folds the
+// same pairs, a_middle - h < j <= a_middle: the reference walks back from
+// a_middle while a view's ray lies in the voxel's PI window (|t| <= P/4) and
+// then forward to half a turn after the first view outside it, so of each
+// pair the voxel takes view j when its ray is in the window and view j + h
+// otherwise (exact whenever the window test does not change sign twice along
+// the walk).  Its rows are linear in z as the helical ones, and the window
+// table marks the slices where every column of a tile takes the same view.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -31,8 +40,6 @@
 #include "lct_internal.hpp"
 
 namespace lct {
-
-long long count_updates(const lct_plan &plan);
 
 constexpr int kRowPad = 4;  // zero row slots before row 0
 // Backprojector tile: 16 (x) x 32 (y) voxel columns per 512-thread workgroup.
@@ -160,6 +167,7 @@ struct FoldBpArgs {
     size_t part_stride;  // floats per call volume
     const int *slab_tab;       // per block: qoff, slice0, a_middle[32]
     int n_pairs;               // pairs j of a block: a_middle[0] - h <= j < a_middle[31]
+                               // (PI-original: a_middle[0] - h < j <= a_middle[31])
     float *vol;
     int n_tiles_x, n_tiles_y, n_wg, xcd_remap;
     int nx, ny, h;
@@ -253,6 +261,11 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 // `shift` = (start mod 4) floats into its LDS region.  The two regions are
 // 4 Up floats apart (Up = units per view rounded up to 64), so each wave's
 // 64 units belong to one view and fill 1 KB of LDS in lane order.
+//
+// PI (the PI-original backprojector, Backprojector.cu:171-250): channels
+// s / ds + ws, rows from t = root (zc - zr) / lhat +- gamma P / 2 pi, one view
+// of each pair per voxel (see the file header); capped at 6 waves per SIMD
+// like the f32 register-staged form (68 VGPRs for 16-bit storage, no spills).
 template <typename T, int NPRE, int LP, bool G = false, bool D = false, bool PI = false>
 __global__ __launch_bounds__(kBpThreads)
 __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !D) || PI ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
