@@ -86,6 +86,43 @@ def test_large_detector_sampled_parity(ctx, large, large_ref, plain_volume, pres
         assert rmse_hu <= 1.0
 
 
+PI_SLICES = [3, 28]
+
+
+@pytest.fixture(scope="module")
+def large_ref_pi(large):
+    w, fan, a, c, _ = large
+    opl, _, _ = oracle_plan(w)
+    fan_np = fan.cpu().numpy()
+    print(f"oracle: PI-original block 0 slices {PI_SLICES}", flush=True)
+    v = O.reconstruct(opl, fan_np, a, c, rebin_kind=2, weight_kind=1, bp_kind=1,
+                      window=w.window, blocks=(0, 1), iz_list=PI_SLICES)
+    del fan_np
+    return np.stack([v[z] for z in PI_SLICES])
+
+
+@pytest.mark.parametrize("preset", ["fast_f32_pi_ori", "mixed_f16_pi_ori"])
+def test_large_detector_pi_original(ctx, large, large_ref_pi, preset):
+    """The PI-original stage set with the folded backprojector at the
+    wide-window instantiation (LDS pitch 105) against the oracle's
+    PI-original reconstruction of block 0: FP32 <= 1e-4, f16 <= 1 HU."""
+    w, fan, a, c, pl = large
+    print(f"large detector: {preset}", flush=True)
+    vol = ctx.reconstruct(pl, recon.preset(preset, window=w.window), a, c, fan)
+    ctx.synchronize()
+    got = np.stack([vol[z].cpu().numpy() for z in PI_SLICES])
+    del vol
+    ref = large_ref_pi
+    assert np.isfinite(ref).all()
+    err = rel_rmse(got, ref)
+    rmse_hu, max_hu = hu_errors(got, ref)
+    print(f"  rel rmse {err:.2e}  rmse {rmse_hu:.4f} HU  max {max_hu:.3f} HU", flush=True)
+    if preset.startswith("fast_f32"):
+        assert err <= 1e-4
+    else:
+        assert rmse_hu <= 1.0
+
+
 def test_large_detector_paths_agree(ctx, large, plain_volume):
     """Over the whole 1024^2 x 512 volume: the pair-folded resident
     backprojector against the plain per-voxel path.  Different kernels, no
