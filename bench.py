@@ -418,6 +418,8 @@ def large_workload(ctx, args, recon, scan, torch):
                                               r["updates"], r["bp_launches_per_volume"],
                                               r["bp_kernel_ms_per_launch"] * 1e-3,
                                               4 if storage == "f32" else 2, n_vox)}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"], res["hu_vs_fp64_oracle"] = large_cpu_baseline(wl, fan, a, c, vol)
     if args.large_slab > 0:
         host_fan = torch.empty(fan.shape, dtype=fan.dtype, pin_memory=True)
         host_fan.copy_(fan)
@@ -434,6 +436,48 @@ def large_workload(ctx, args, recon, scan, torch):
     del fan, vol
     torch.cuda.empty_cache()
     return res
+
+
+def large_cpu_baseline(wl, fan, a, c, vol, sample_slices=(5, 22)):
+    """FP64 oracle on all host threads for the large workload, extrapolated:
+    block 0's window rebinned and filtered and `sample_slices` of its 32
+    slices backprojected, scaled to every block x 32 slices; and the GPU
+    volume's error against the oracle on that sample."""
+    import numpy as np
+    from listctrecons_amd import scan
+    from oracle import oracle as O
+    p = O.Prm()
+    pp = wl.prm()
+    for f, _ in O.Prm._fields_:
+        setattr(p, f, getattr(pp, f))
+    pl = O.plan_helical(p, a, c)
+    s, _, _, _ = O.block_info(pl, c, 0)
+    n_views = s + pl.data_offset + pl.ctg.n_angle_add  # the views block 0 reads
+    fan_np = fan[:n_views].cpu().numpy()
+    threads = O.num_threads()
+    log(f"  FP64 oracle on the host: block 0, slices {list(sample_slices)}")
+    t0 = time.perf_counter()
+    O.reconstruct(pl, fan_np, a, c, window=wl.window, blocks=(0, 1), iz_list=[])
+    t_rf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ovol = O.reconstruct(pl, fan_np, a, c, window=wl.window, blocks=(0, 1),
+                         iz_list=list(sample_slices))
+    t_all = time.perf_counter() - t0
+    t_slice = max(t_all - t_rf, 0.0) / len(sample_slices)
+    idx = list(sample_slices)
+    ref = ovol[idx]
+    got = vol[idx].cpu().numpy()
+    err = hu_stats(got, ref, np, scan)
+    err["rel_rmse"] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    err["sample"] = f"block 0 slices {idx}, all voxels"
+    cb = {"value": round(pl.n_blocks * (t_rf + 32 * t_slice), 3), "unit": "s/volume",
+          "cores": threads, "kind": "port", "extrapolated": True, "cpu": cpu_model(),
+          "sample": (f"FP64 oracle, {threads} threads, block 0 of {pl.n_blocks}: rebin+filter of "
+                     f"its {pl.ctg.n_angle_add}-view window ({t_rf:.2f} s) and {len(idx)} of 32 "
+                     f"slices backprojected ({t_slice:.2f} s/slice), extrapolated to "
+                     f"{pl.n_blocks} blocks x 32 slices"),
+          "sample_seconds": round(t_rf + t_all, 3)}
+    return cb, err
 
 
 def hu_stats(x, ref, np, scan):

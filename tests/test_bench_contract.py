@@ -94,3 +94,28 @@ def test_roofline_recomputes_from_the_counters():
             prof["sq_lds_idx_active_per_launch"] / 256 / (launch_s * 2.4e9), abs=1e-3)
         assert rf["traffic_frac_of_hbm_peak"] == pytest.approx(
             prof["traffic_bytes_per_launch"] / launch_s / 8e12, abs=1e-4)
+
+
+def test_large_cpu_baseline_is_labelled_extrapolated():
+    """The large workload's CPU figure: block 0's window rebinned and filtered
+    and two sampled slices backprojected by the FP64 oracle, extrapolated to
+    the whole volume and labelled so; the volume's error on that sample (here
+    the oracle's own slices in float32: rounding only)."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import bench
+    from helpers import cpu_fan, oracle_plan
+    from listctrecons_amd import scan
+    from oracle import oracle as O
+    w = scan.workload("helical_16x368_256x32")
+    pl, a, c = oracle_plan(w)
+    fan = cpu_fan(w)
+    ovol = O.reconstruct(pl, fan, a, c, window=w.window, blocks=(0, 1), iz_list=[5, 22])
+    vol = torch.from_numpy(np.nan_to_num(ovol).astype(np.float32))
+    cb, err = bench.large_cpu_baseline(w, torch.from_numpy(fan), a, c, vol)
+    assert cb["extrapolated"] is True and cb["kind"] == "port" and cb["value"] > 0
+    assert cb["cores"] >= 1 and "extrapolated to 1 blocks x 32 slices" in cb["sample"]
+    assert err["rmse"] < 1e-3 and err["rel_rmse"] < 1e-6
+    assert err["sample"] == "block 0 slices [5, 22], all voxels"
