@@ -494,18 +494,26 @@ __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !D) || PI ? 6 : 1))) void 
                 const float lo = fmaf(ap, v10 - v00, v00), hi = fmaf(ap, v11 - v01, v01);
                 return fmaf(ar, hi - lo, lo);
             };
+            // One loop per slice class, each with one scalar bit test per slice; a
+            // slice is in one class, so each is still summed once, in pair order.
+            // (Tested in one loop as an if / else-if chain, the three classes
+            // shared one tail behind register moves and flow-control branches:
+            // 22.3 -> 21.1 ms on the clinical volume, 279 -> 245 ms on the large one.)
+            const unsigned m1 = zm & fm, m2 = zm & ~fm & fm2, mx = zm & ~fm & ~fm2;
 #pragma unroll
-            for (int z = 0; z < 32; z++) {
-                const unsigned bit = 1u << z;
-                if (zm & bit) {
-                    if (fm & bit) {
-                        acc[z] += tap(b1, g1, ap1, fmaf(float(z), K1, R1));
-                    } else if (fm2 & bit) {
-                        acc[z] += tap(b2, g2, ap2, fmaf(float(z), K2, R2));
-                    } else {
+            for (int z = 0; z < 32; z++)
+                if (m1 & (1u << z)) acc[z] += tap(b1, g1, ap1, fmaf(float(z), K1, R1));
+            if (m2) {
+#pragma unroll
+                for (int z = 0; z < 32; z++)
+                    if (m2 & (1u << z)) acc[z] += tap(b2, g2, ap2, fmaf(float(z), K2, R2));
+            }
+            if (mx) {
+#pragma unroll
+                for (int z = 0; z < 32; z++) {
+                    if (mx & (1u << z)) {
                         // per column: the view first, then its one tap (rows
                         // clamped to the zero guard slots: the border)
-                        asm volatile("");
                         const float r1 = fmaf(float(z), K1, R1), r2 = fmaf(float(z), K2, R2);
                         const bool in1 = fabsf(r1 - row0) <= A.wt;
                         acc[z] += tap(in1 ? b1 : b2, in1 ? g1 : g2, in1 ? ap1 : ap2,
