@@ -264,11 +264,15 @@ __device__ __forceinline__ int window_start(float c, int H, int cp, int W)
 //
 // PI (the PI-original backprojector, Backprojector.cu:171-250): channels
 // s / ds + ws, rows from t = root (zc - zr) / lhat +- gamma P / 2 pi, one view
-// of each pair per voxel (see the file header); capped at 6 waves per SIMD
-// like the f32 register-staged form (68 VGPRs for 16-bit storage, no spills).
+// of each pair per voxel (see the file header).  Its slices carry one tap, so
+// latency rather than issue limits it: it is held to 64 VGPRs, 8 waves per
+// SIMD (4 workgroups per CU instead of 3 at 67 VGPRs; the two 8-byte values
+// spilled are stored before the pair loop and reloaded after it): 21.0 ->
+// 18.6 ms on the clinical volume, 245.7 -> 219.5 ms on the large one.  The
+// helical form would spill 15 registers at 64, and keeps 6 waves.
 template <typename T, int NPRE, int LP, bool G = false, bool D = false, bool PI = false>
 __global__ __launch_bounds__(kBpThreads)
-__attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !D) || PI ? 6 : 1))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
+__attribute__((amdgpu_waves_per_eu(PI && !G ? 8 : ((sizeof(T) == 4 && !D) ? 6 : 1)))) void bp_fold_kernel(FoldBpArgs A, const float2 *__restrict__ cs,
                                                       const unsigned *__restrict__ zmask,
                                                       const uint2 *__restrict__ win)
 {
