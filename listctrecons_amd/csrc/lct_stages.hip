@@ -12,6 +12,7 @@
 #include <cmath>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lct_internal.hpp"
@@ -30,15 +31,19 @@ template <int V>
 __global__ __launch_bounds__(256) void rebin_views_lerp_kernel(
     const float *__restrict__ fan, long fan_views, float *__restrict__ par, long base, int ns,
     int nt, const int *__restrict__ k0, const float *__restrict__ fr,
-    const float *__restrict__ wtab, long n_out)
+    const float *__restrict__ wtab, long n_out, int rows)
 {
-    // 1-D grid (a multiple of 8) of (channel chunk, group of V output views);
-    // each XCD takes a contiguous range of groups (their fan views share its L2)
+    // 1-D grid (a multiple of 8) of (channel chunk, chunk of `rows` rows, group
+    // of V output views); each XCD takes a contiguous range of groups (their fan
+    // views share its L2)
     const long n_wg = long(gridDim.x);
     const long wg = long(blockIdx.x & 7u) * (n_wg >> 3) + long(blockIdx.x >> 3);
-    const long n_cx = (ns + 255) / 256;
-    const long grp = wg / n_cx;
-    const int is = int(wg - grp * n_cx) * 256 + int(threadIdx.x);
+    const long n_cx = (ns + 255) / 256, n_rc = (nt + rows - 1) / rows;
+    const long grp = wg / (n_cx * n_rc);
+    const long rem = wg - grp * n_cx * n_rc;
+    const long rc = rem / n_cx;
+    const int is = int(rem - rc * n_cx) * 256 + int(threadIdx.x);
+    const int it0 = int(rc) * rows, it1 = min(nt, it0 + rows);
     const long ia0 = grp * V;
     if (ia0 >= n_out || is >= ns) return;
     const int nv = int(n_out - ia0 < V ? n_out - ia0 : V);
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(256) void rebin_views_lerp_kernel(
     }
     float *o = par + size_t(ia0) * view + is;
 #pragma unroll 4
-    for (int it = 0; it < nt; it++) {
+    for (int it = it0; it < it1; it++) {
         const size_t r = size_t(it) * ns;
         float x[V + 1];
 #pragma unroll
@@ -74,6 +79,9 @@ __global__ __launch_bounds__(256) void rebin_views_lerp_kernel(
 // Output views per thread of the rebin (clinical scan: 1.24 ms with 1,
 // 0.99 with 2, 0.82 with 4, 0.71 with 8, 0.72 with 16).
 constexpr int kRebinViews = 8;
+// Detector rows per thread of the rebin: rows are independent, and a thread
+// walking all of them left too few waves to hide the loads' latency.
+constexpr int kRebinRows = 16;
 
 // CUDA/Rebiner.cu:156-179 (PI_ORI): trilinear sample at (pI, qI, bI), with
 // pI and the view fraction depending on the channel only and qI on (row,
@@ -87,13 +95,17 @@ __global__ __launch_bounds__(256) void rebin_pi_ori_kernel(
     const float *__restrict__ fan, long fan_views, float *__restrict__ par, long base, int ns,
     int nt, const int *__restrict__ pi0, const float *__restrict__ pfr,
     const int *__restrict__ vk0, const float *__restrict__ vfr, const float *__restrict__ qc1,
-    const float *__restrict__ qc0, float wt, const float *__restrict__ wtab, long n_out)
+    const float *__restrict__ qc0, float wt, const float *__restrict__ wtab, long n_out, int rows)
 {
+    // grid as in rebin_views_lerp_kernel: (channel chunk, row chunk, view group)
     const long n_wg = long(gridDim.x);
     const long wg = long(blockIdx.x & 7u) * (n_wg >> 3) + long(blockIdx.x >> 3);
-    const long n_cx = (ns + 255) / 256;
-    const long grp = wg / n_cx;
-    const int is = int(wg - grp * n_cx) * 256 + int(threadIdx.x);
+    const long n_cx = (ns + 255) / 256, n_rc = (nt + rows - 1) / rows;
+    const long grp = wg / (n_cx * n_rc);
+    const long rem = wg - grp * n_cx * n_rc;
+    const long rc = rem / n_cx;
+    const int is = int(rem - rc * n_cx) * 256 + int(threadIdx.x);
+    const int it0 = int(rc) * rows, it1 = min(nt, it0 + rows);
     const long ia0 = grp * V;
     if (ia0 >= n_out || is >= ns) return;
     const int nv = int(n_out - ia0 < V ? n_out - ia0 : V);
@@ -114,34 +126,57 @@ __global__ __launch_bounds__(256) void rebin_pi_ori_kernel(
     }
     const long d0 = long(x0) - xo, d1 = d0 + 1;  // channel offsets of the two taps from xo
     float *o = par + size_t(ia0) * view + is;
-    for (int it = 0; it < nt; it++) {
-        const float qI = fmaf(float(it) - wt, c1, c0);
-        const float fy = floorf(qI);
-        const int y0 = int(fy);
-        const float ay = qI - fy;
-        const bool oky0 = y0 >= 0 && y0 < nt, oky1 = y0 + 1 >= 0 && y0 + 1 < nt;
-        const long r0 = long(y0) * ns, r1 = r0 + ns;
-        float smp[V + 1];
+    // The lanes of a wave take consecutive channels, whose fan views differ
+    // (the view offset moves with the fan angle), so each load touches many
+    // cache lines.  When both taps of every lane lie in range (all waves but
+    // the border ones), the two taps, adjacent floats, come in one 8-byte load.
+    auto rows_loop = [&](auto pair_loads) {
+        for (int it = it0; it < it1; it++) {
+            const float qI = fmaf(float(it) - wt, c1, c0);
+            const float fy = floorf(qI);
+            const int y0 = int(fy);
+            const float ay = qI - fy;
+            const bool oky0 = y0 >= 0 && y0 < nt, oky1 = y0 + 1 >= 0 && y0 + 1 < nt;
+            const long r0 = long(y0) * ns, r1 = r0 + ns;
+            float smp[V + 1];
 #pragma unroll
-        for (int j = 0; j <= V; j++) {
-            const float *f = fan + offz[j];
-            const float v00 = okz[j] && oky0 && okx0 ? f[r0 + d0] : 0.0f;
-            const float v01 = okz[j] && oky0 && okx1 ? f[r0 + d1] : 0.0f;
-            const float v10 = okz[j] && oky1 && okx0 ? f[r1 + d0] : 0.0f;
-            const float v11 = okz[j] && oky1 && okx1 ? f[r1 + d1] : 0.0f;
-            const float lo = fmaf(ax, v01 - v00, v00), hi = fmaf(ax, v11 - v10, v10);
-            smp[j] = fmaf(ay, hi - lo, lo);
-        }
-        const size_t r = size_t(it) * ns;
-        const float w = wtab ? wtab[r + is] : 1.0f;
+            for (int j = 0; j <= V; j++) {
+                const float *f = fan + offz[j];
+                float v00, v01, v10, v11;
+                if constexpr (decltype(pair_loads)::value) {
+                    // d0 == 0 here: the pair starts at xo = x0 (4-byte aligned)
+                    typedef float float2u __attribute__((ext_vector_type(2), aligned(4)));
+                    const float2u z2 = {0.0f, 0.0f};
+                    const float2u p0 = okz[j] && oky0 ? *reinterpret_cast<const float2u *>(f + r0) : z2;
+                    const float2u p1 = okz[j] && oky1 ? *reinterpret_cast<const float2u *>(f + r1) : z2;
+                    v00 = p0.x;
+                    v01 = p0.y;
+                    v10 = p1.x;
+                    v11 = p1.y;
+                } else {
+                    v00 = okz[j] && oky0 && okx0 ? f[r0 + d0] : 0.0f;
+                    v01 = okz[j] && oky0 && okx1 ? f[r0 + d1] : 0.0f;
+                    v10 = okz[j] && oky1 && okx0 ? f[r1 + d0] : 0.0f;
+                    v11 = okz[j] && oky1 && okx1 ? f[r1 + d1] : 0.0f;
+                }
+                const float lo = fmaf(ax, v01 - v00, v00), hi = fmaf(ax, v11 - v10, v10);
+                smp[j] = fmaf(ay, hi - lo, lo);
+            }
+            const size_t r = size_t(it) * ns;
+            const float w = wtab ? wtab[r + is] : 1.0f;
 #pragma unroll
-        for (int j = 0; j < V; j++) {
-            if (j >= nv) break;
-            float v = fmaf(az, smp[j + 1] - smp[j], smp[j]);
-            if (wtab) v *= w;
-            o[size_t(j) * view + r] = v;
+            for (int j = 0; j < V; j++) {
+                if (j >= nv) break;
+                float v = fmaf(az, smp[j + 1] - smp[j], smp[j]);
+                if (wtab) v *= w;
+                o[size_t(j) * view + r] = v;
+            }
         }
-    }
+    };
+    if (__builtin_amdgcn_ballot_w64(!(okx0 && okx1)) == 0)
+        rows_loop(std::integral_constant<bool, true>());
+    else
+        rows_loop(std::integral_constant<bool, false>());
 }
 
 // CUDA/PreWeighting.cu:8-29: par[ia][it][is] *= w[it][is].
@@ -220,22 +255,23 @@ int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *gp, cons
 
     const float *fan = d_fan + size_t(fan_view0) * ns * nt;
     if (n_out <= 0) return LCT_OK;
+    const long n_rc = (nt + kRebinRows - 1) / kRebinRows;
     if (kind == LCT_REBIN_PI_ORI) {
         constexpr int V = kRebinViews;
         const long n_grp = (n_out + V - 1) / V;
-        const long n_wg = ((long((ns + 255) / 256) * n_grp) + 7) & ~7L;
+        const long n_wg = ((long((ns + 255) / 256) * n_rc * n_grp) + 7) & ~7L;
         if (n_wg > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "rebin: too many views");
         hipLaunchKernelGGL(rebin_pi_ori_kernel<V>, dim3(unsigned(n_wg)), dim3(256), 0, ctx->stream,
                            fan, fan_views, d_par, out_view0, ns, nt, d_i, d_f, d_i + ns, d_f + ns,
-                           d_f + 2 * ns, d_f + 3 * ns, g.wt, d_w, long(n_out));
+                           d_f + 2 * ns, d_f + 3 * ns, g.wt, d_w, long(n_out), kRebinRows);
     } else {
         constexpr int V = kRebinViews;
         const long n_grp = (n_out + V - 1) / V;
-        const long n_wg = ((long((ns + 255) / 256) * n_grp) + 7) & ~7L;
+        const long n_wg = ((long((ns + 255) / 256) * n_rc * n_grp) + 7) & ~7L;
         if (n_wg > 0x7fffffffL) return set_error(LCT_ERR_UNSUPPORTED, "rebin: too many views");
         hipLaunchKernelGGL(rebin_views_lerp_kernel<V>, dim3(unsigned(n_wg)), dim3(256), 0,
                            ctx->stream, fan, fan_views, d_par, out_view0, ns, nt, d_i, d_f, d_w,
-                           long(n_out));
+                           long(n_out), kRebinRows);
     }
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
