@@ -499,7 +499,7 @@ def main():
                     help="skip timing the learned correction's inference")
     ap.add_argument("--no-pi-ori", action="store_true",
                     help="skip the PI-original stage set (per-block FP32)")
-    ap.add_argument("--variants", default="fast_f32,mixed_bf16",
+    ap.add_argument("--variants", default="fast_f32,fast_f32_x6,mixed_bf16",
                     help="comma-separated presets to time besides the headline path "
                          "(e.g. fast_f32,fast_f32_gather,mixed_f16_gather,mixed_bf16)")
     ap.add_argument("--stream-slabs", type=int, default=2,

@@ -45,7 +45,7 @@ enum { LCT_F32 = 0, LCT_F16 = 1, LCT_BF16 = 2 };
 enum { LCT_REBIN_NULL = 0, LCT_REBIN_FAN2PAR = 1, LCT_REBIN_PI_ORI = 2 };
 enum { LCT_WEIGHT_NONE = 0, LCT_WEIGHT_PI_ORI = 1 };
 enum { LCT_WINDOW_NONE = 0, LCT_WINDOW_HANN = 1, LCT_WINDOW_SHEPP_LOGAN = 2 };
-enum { LCT_FILTER_DIRECT_F32 = 0, LCT_FILTER_TOEPLITZ_BF16X3 = 1 };
+enum { LCT_FILTER_DIRECT_F32 = 0, LCT_FILTER_TOEPLITZ_BF16X3 = 1, LCT_FILTER_TOEPLITZ_BF16X6 = 2 };
 enum { LCT_BP_HELICAL = 0, LCT_BP_PI_ORI = 1 };
 /* Bilinear taps of the backprojector: LDS = view windows staged in LDS and
  * interpolated explicitly; GATHER = taps loaded from HBM through the vector
@@ -166,10 +166,12 @@ int lct_filter(lct_ctx *ctx, const lct_ct_geom *g, const float *h_taps, float *d
  * the resident backprojector's layout: d_rows [views][nt][ns] f32 ->
  * d_q [views][ns + 3][nt + 8] of `storage` type, channel-major, channel c at
  * slot c + 1 and row r at slot r + 4, every other slot 0 (the caller clears
- * d_q).  algo LCT_FILTER_DIRECT_F32 (FP32 FMA, k ascending) or
- * LCT_FILTER_TOEPLITZ_BF16X3 (Toeplitz GEMM on BF16 MFMA, hi/lo split
- * operands, FP32 accumulation).  Needs nt % 8 == 0 (and ns % 4 == 0 for the
- * Toeplitz algorithm). */
+ * d_q).  algo LCT_FILTER_DIRECT_F32 (FP32 FMA, k ascending),
+ * LCT_FILTER_TOEPLITZ_BF16X3 (Toeplitz GEMM on BF16 MFMA, taps and data split
+ * in two bf16 parts, three products, FP32 accumulation) or
+ * LCT_FILTER_TOEPLITZ_BF16X6 (both split in three parts, six products:
+ * FP32-level accuracy).  Needs nt % 8 == 0 (and ns % 4 == 0 for the Toeplitz
+ * algorithms). */
 int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int ns, int nt,
                             const float *h_taps, const float *d_rows, long views, void *d_q);
 

@@ -22,7 +22,7 @@ F32, F16, BF16 = 0, 1, 2
 REBIN_NULL, REBIN_FAN2PAR, REBIN_PI_ORI = 0, 1, 2
 WEIGHT_NONE, WEIGHT_PI_ORI = 0, 1
 WINDOW_NONE, WINDOW_HANN, WINDOW_SHEPP_LOGAN = 0, 1, 2
-FILTER_DIRECT_F32, FILTER_TOEPLITZ_BF16X3 = 0, 1
+FILTER_DIRECT_F32, FILTER_TOEPLITZ_BF16X3, FILTER_TOEPLITZ_BF16X6 = 0, 1, 2
 BP_HELICAL, BP_PI_ORI = 0, 1
 INTERP_LDS, INTERP_GATHER = 0, 1
 PIPE_BLOCKS, PIPE_RESIDENT = 0, 1

@@ -256,6 +256,8 @@ void FDKResidentReconstructor::reconstruct(const ReconData *mr)
 //   "null"         NullRebinerGPU
 //   "hann"/"shepp" apodisation window of RampFilterGPU
 //   "resident"     FDKResidentReconstructor, FP32 storage
+//   "x6"           ... FP32 storage with the six-product BF16 Toeplitz MFMA
+//                  ramp filter (FP32-level accuracy) instead of the FP32 FMA one
 //   "f16"/"bf16"   FDKResidentReconstructor, f16 / bf16 filtered projections
 //                  with the BF16x3 Toeplitz MFMA ramp filter
 //   "slab<N>"      FDKResidentReconstructor streaming slabs of N blocks
@@ -272,7 +274,7 @@ Reconstructor *ReconstrctorAbstractFactory::createReconstructor(const std::strin
         else if (has(s, "shepp")) win = SmoothWindow::SHEPP_LOGAN;
         const bool piori = has(s, "piori");
         const bool null_rebin = has(s, "null");
-        if (has(s, "resident") || has(s, "f16") || has(s, "slab") || has(s, "ckpt")) {
+        if (has(s, "resident") || has(s, "f16") || has(s, "x6") || has(s, "slab") || has(s, "ckpt")) {
             lct_recon_opts o{};
             o.pipeline = LCT_PIPE_RESIDENT;
             o.rebin_kind = piori ? LCT_REBIN_PI_ORI : (null_rebin ? LCT_REBIN_NULL : LCT_REBIN_FAN2PAR);
@@ -280,7 +282,9 @@ Reconstructor *ReconstrctorAbstractFactory::createReconstructor(const std::strin
             o.bp_kind = piori ? LCT_BP_PI_ORI : LCT_BP_HELICAL;
             o.window = int(win);
             o.storage = has(s, "bf16") ? LCT_BF16 : (has(s, "f16") ? LCT_F16 : LCT_F32);
-            o.filter_algo = o.storage == LCT_F32 ? LCT_FILTER_DIRECT_F32 : LCT_FILTER_TOEPLITZ_BF16X3;
+            o.filter_algo = o.storage != LCT_F32 ? LCT_FILTER_TOEPLITZ_BF16X3
+                            : has(s, "x6")       ? LCT_FILTER_TOEPLITZ_BF16X6
+                                                 : LCT_FILTER_DIRECT_F32;
             o.interp = LCT_INTERP_LDS;
             auto *r = new FDKResidentReconstructor(o);
             const bool ckpt = has(s, "ckpt");

@@ -110,9 +110,10 @@ int launch_rebin(lct_ctx *ctx, int kind, int weight, const lct_ct_geom *g, const
                  int transposed);
 int launch_weighting(lct_ctx *ctx, const lct_ct_geom *g, float *d_par);
 int launch_filter_direct(lct_ctx *ctx, int ns, const float *d_taps, float *d_rows, long rows);
-// BF16x3 Toeplitz-GEMM ramp filter of rows [n_cols][ns] f32 into the
-// backprojector layout Q[view][cp][rp] of the given storage type.
-int launch_filter_toeplitz(lct_ctx *ctx, int storage, int ns, int nt, const float *h_taps,
+// Toeplitz-GEMM ramp filter on BF16 MFMA (algo LCT_FILTER_TOEPLITZ_BF16X3 or
+// _BF16X6) of rows [n_cols][ns] f32 into the backprojector layout
+// Q[view][cp][rp] of the given storage type.
+int launch_filter_toeplitz(lct_ctx *ctx, int algo, int storage, int ns, int nt, const float *h_taps,
                            const float *d_in, void *d_q, long n_cols, int cp, int rp);
 // Literal per-voxel backprojection of one 32-slice block (tables uploaded here).
 int backproject_block_public(lct_ctx *ctx, int kind, const lct_ct_geom &g,

@@ -132,7 +132,7 @@ static int reconstruct_range(lct_ctx *ctx, const lct_plan *plan, const lct_recon
     const lct_recon_opts &o = *opts;
     if (o.rebin_kind < 0 || o.rebin_kind > 2 || o.weight_kind < 0 || o.weight_kind > 1 ||
         o.window < 0 || o.window > 2 || o.bp_kind < 0 || o.bp_kind > 1 || o.storage < 0 ||
-        o.storage > 2 || o.filter_algo < 0 || o.filter_algo > 1 || o.interp < 0 || o.interp > 1)
+        o.storage > 2 || o.filter_algo < 0 || o.filter_algo > 2 || o.interp < 0 || o.interp > 1)
         return set_error(LCT_ERR_INVALID, std::string(who) + ": option out of range");
     if (plan->n_blocks <= 0 || plan->ig.nz != LCT_BLOCK_SLICES || plan->ctg.n_angle <= 0)
         return set_error(LCT_ERR_INVALID, std::string(who) + ": plan has no blocks");

@@ -1044,9 +1044,9 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
         const long rows = (v1 - v0) * g.nt;
         const float *dt = static_cast<const float *>(d_taps);
         LCT_CHECK(tic());
-        if (o.filter_algo == LCT_FILTER_TOEPLITZ_BF16X3 && g.ns % 4 == 0)
-            LCT_CHECK(launch_filter_toeplitz(ctx, o.storage, g.ns, g.nt, taps.data(), in, q, rows,
-                                             cp, rp));
+        if (o.filter_algo != LCT_FILTER_DIRECT_F32 && g.ns % 4 == 0)
+            LCT_CHECK(launch_filter_toeplitz(ctx, o.filter_algo, o.storage, g.ns, g.nt, taps.data(),
+                                             in, q, rows, cp, rp));
         else if (o.storage == LCT_F32)
             LCT_CHECK(launch_filter_to_q<float>(ctx, g.ns, g.nt, dt, in, static_cast<float *>(q),
                                                 rows, cp, rp));
@@ -1251,7 +1251,7 @@ int reconstruct_resident(lct_ctx *ctx, const lct_plan &plan, const lct_recon_opt
     // 40.0 -> 38.4 ms per volume). Against the 2 ms Toeplitz filter the split
     // backprojector launches cost more than they hide (32.1 -> 32.6-33.7 ms
     // with f16 storage, 414 -> 420 ms on the large volume).
-    const bool toeplitz = o.filter_algo == LCT_FILTER_TOEPLITZ_BF16X3 && g.ns % 4 == 0;
+    const bool toeplitz = o.filter_algo != LCT_FILTER_DIRECT_F32 && g.ns % 4 == 0;
     if (ctx->profiling || nb < 2 || toeplitz) {
         // one stage after the other on the context stream (each one timed)
         LCT_CHECK(prep(0, R.nviews));
@@ -1358,15 +1358,15 @@ extern "C" int lct_filter_to_bp_layout(lct_ctx *ctx, int algo, int storage, int 
                                        void *d_q)
 {
     if (!ctx || !h_taps || !d_rows || !d_q || ns <= 0 || nt <= 0 || nt % 8 != 0 || views < 0 ||
-        storage < 0 || storage > 2 || algo < 0 || algo > 1)
+        storage < 0 || storage > 2 || algo < 0 || algo > 2)
         return set_error(LCT_ERR_INVALID, "lct_filter_to_bp_layout: bad argument");
-    if (algo == LCT_FILTER_TOEPLITZ_BF16X3 && ns % 4 != 0)
+    if (algo != LCT_FILTER_DIRECT_F32 && ns % 4 != 0)
         return set_error(LCT_ERR_UNSUPPORTED, "lct_filter_to_bp_layout: Toeplitz needs ns % 4 == 0");
     LCT_CHECK_HIP(hipSetDevice(ctx->device));
     const int cp = ns + 3, rp = nt + 8;
     const long rows = views * nt;
-    if (algo == LCT_FILTER_TOEPLITZ_BF16X3)
-        return launch_filter_toeplitz(ctx, storage, ns, nt, h_taps, d_rows, d_q, rows, cp, rp);
+    if (algo != LCT_FILTER_DIRECT_F32)
+        return launch_filter_toeplitz(ctx, algo, storage, ns, nt, h_taps, d_rows, d_q, rows, cp, rp);
     void *d_taps = nullptr;
     LCT_CHECK(scratch(ctx, kScratchTaps, size_t(2) * ns * sizeof(float), &d_taps));
     LCT_CHECK(upload(ctx, d_taps, h_taps, size_t(2) * ns * sizeof(float)));

@@ -7,7 +7,13 @@
 // of every view).  The taps are split two ways, h = h1 + h2 (bf16 each), the
 // data two ways, x = x1 + x2 (~2^-17), and
 //   h1 x1 + h1 x2 + h2 x1
-// is accumulated in FP32 with three v_mfma_f32_16x16x32_bf16 per fragment.
+// is accumulated in FP32 with three v_mfma_f32_16x16x32_bf16 per fragment
+// (LCT_FILTER_TOEPLITZ_BF16X3).  LCT_FILTER_TOEPLITZ_BF16X6 splits both three
+// ways, h = h1 + h2 + h3 and x = x1 + x2 + x3 (each exact: 3 x 8 bits cover an
+// FP32 mantissa), and accumulates the six products of order <= 2,
+//   h1 x1 + h1 x2 + h2 x1 + h1 x3 + h2 x2 + h3 x1,
+// dropping terms of order 2^-24 relative: FP32-level accuracy at twice the
+// MFMA work of the three-product form.
 // The tap split's error is the same for every row, so it would bias the
 // filter's DC response: h2 is rounded with error diffusion along the tap
 // index (each tap's residual carried into the next one's rounding), so the
@@ -63,13 +69,17 @@ template <> __device__ __forceinline__ __hip_bfloat16 to_store<__hip_bfloat16>(f
     return __float2bfloat16(x);
 }
 
-// 4 waves per SIMD (at most 128 registers, the accumulators in VGPRs, no spills)
-template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
+// NP operand parts (2: three products, 3: six products).  NP = 2 keeps 4
+// waves per SIMD (at most 128 registers, the accumulators in VGPRs, no
+// spills); NP = 3 holds a third A fragment pair and B part, at 3 waves.
+template <typename T, int NP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 2 ? 4 : 3))) void
 toeplitz_filter_kernel(ToeplitzArgs A)
 {
+    static_assert(NP == 2 || NP == 3, "two or three operand parts");
     __shared__ __attribute__((aligned(16))) unsigned short bh[2 * kTN * kBRow];
     __shared__ __attribute__((aligned(16))) unsigned short bl[2 * kTN * kBRow];
+    __shared__ __attribute__((aligned(16))) unsigned short b3[NP == 3 ? 2 * kTN * kBRow : 8];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -106,20 +116,28 @@ toeplitz_filter_kernel(ToeplitzArgs A)
             for (int j = 0; j < 8; j++) x[j] = 0.0f;
         }
     };
-    // x = hi + lo: both rounded to nearest even by the packed hardware
-    // conversion (two values per v_cvt_pk_bf16_f32); x - hi is exact in FP32
+    // x = hi + lo (+ third): each part rounded to nearest even by the packed
+    // hardware conversion (two values per v_cvt_pk_bf16_f32); every
+    // remainder is exact in FP32
     auto commit_b = [&](int buf, const float (&x)[8]) {
-        unsigned hi[4], lo[4];
+        unsigned hi[4], lo[4], th[4];
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const f32x2 v = {x[2 * e], x[2 * e + 1]};
             const bf16x2 h = __builtin_convertvector(v, bf16x2);
             const f32x2 r = v - __builtin_convertvector(h, f32x2);
+            const bf16x2 l = __builtin_convertvector(r, bf16x2);
             hi[e] = __builtin_bit_cast(unsigned, h);
-            lo[e] = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf16x2));
+            lo[e] = __builtin_bit_cast(unsigned, l);
+            if constexpr (NP == 3) {
+                const f32x2 r2 = r - __builtin_convertvector(l, f32x2);
+                th[e] = __builtin_bit_cast(unsigned, __builtin_convertvector(r2, bf16x2));
+            }
         }
         *reinterpret_cast<uint4 *>(bh + (buf * kTN + bcol) * kBRow + bk) = make_uint4(hi[0], hi[1], hi[2], hi[3]);
         *reinterpret_cast<uint4 *>(bl + (buf * kTN + bcol) * kBRow + bk) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        if constexpr (NP == 3)
+            *reinterpret_cast<uint4 *>(b3 + (buf * kTN + bcol) * kBRow + bk) = make_uint4(th[0], th[1], th[2], th[3]);
     };
 
     const long n_tiles = (A.n_cols + kTN - 1) / kTN;
@@ -135,8 +153,8 @@ toeplitz_filter_kernel(ToeplitzArgs A)
         // index (m - 32 ks + kp - 32) / 16 of the precomputed table. Those of
         // k-step ks + 1 are loaded while ks runs, so the table's L2 latency is
         // not waited on at every step.
-        bf16x8 a1[2], a2[2];
-        auto load_a = [&](int ks, bf16x8 (&p1)[2], bf16x8 (&p2)[2]) {
+        bf16x8 a1[2], a2[2], a3[NP == 3 ? 2 : 1];
+        auto load_a = [&](int ks, bf16x8 (&p1)[2], bf16x8 (&p2)[2], bf16x8 (&p3)[NP == 3 ? 2 : 1]) {
 #pragma unroll
             for (int a = 0; a < 2; a++) {
                 const int m = m0 + (2 * wave + a) * 16;
@@ -144,9 +162,10 @@ toeplitz_filter_kernel(ToeplitzArgs A)
                 const unsigned short *f = A.frag + (di * 64 + lane) * 8;
                 p1[a] = *reinterpret_cast<const bf16x8 *>(f);
                 p2[a] = *reinterpret_cast<const bf16x8 *>(f + A.frag_stride);
+                if constexpr (NP == 3) p3[a] = *reinterpret_cast<const bf16x8 *>(f + 2 * A.frag_stride);
             }
         };
-        load_a(0, a1, a2);
+        load_a(0, a1, a2, a3);
         float xb[8];
         stage_b(n0, 0, xb);
         commit_b(0, xb);
@@ -154,15 +173,23 @@ toeplitz_filter_kernel(ToeplitzArgs A)
         for (int ks = 0; ks < n_ksteps; ks++) {
             const int buf = ks & 1;
             if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, xb);
-            bf16x8 n1[2], n2[2];
-            load_a(ks + 1 < n_ksteps ? ks + 1 : ks, n1, n2);
+            bf16x8 n1[2], n2[2], n3[NP == 3 ? 2 : 1];
+            load_a(ks + 1 < n_ksteps ? ks + 1 : ks, n1, n2, n3);
 #pragma unroll
             for (int f = 0; f < 4; f++) {
                 const int c = f * 16 + li;
                 const bf16x8 b_hi = *reinterpret_cast<const bf16x8 *>(bh + (buf * kTN + c) * kBRow + lk);
                 const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(bl + (buf * kTN + c) * kBRow + lk);
+                bf16x8 b_th;
+                if constexpr (NP == 3) b_th = *reinterpret_cast<const bf16x8 *>(b3 + (buf * kTN + c) * kBRow + lk);
 #pragma unroll
                 for (int a = 0; a < 2; a++) {
+                    // smallest products first
+                    if constexpr (NP == 3) {
+                        acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3[a], b_hi, acc[a][f], 0, 0, 0);
+                        acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[a], b_lo, acc[a][f], 0, 0, 0);
+                        acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_th, acc[a][f], 0, 0, 0);
+                    }
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2[a], b_hi, acc[a][f], 0, 0, 0);
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_lo, acc[a][f], 0, 0, 0);
                     acc[a][f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[a], b_hi, acc[a][f], 0, 0, 0);
@@ -173,6 +200,7 @@ toeplitz_filter_kernel(ToeplitzArgs A)
             for (int a = 0; a < 2; a++) {
                 a1[a] = n1[a];
                 a2[a] = n2[a];
+                if constexpr (NP == 3) a3[a] = n3[a];
             }
             __syncthreads();
         }
@@ -213,7 +241,7 @@ static double host_bf16_f(unsigned short b)
     return f;
 }
 
-template <typename T>
+template <typename T, int NP>
 int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, const float *d_in,
                              T *d_q, long n_cols, int cp, int rp)
 {
@@ -222,18 +250,25 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
     // diagonals d = m - k0 from -(kp - 32) to mp - 16 in steps of 16
     const int n_diag = (mp + kp - kTK) / 16;
     const long stride = long(n_diag) * 64 * 8;
-    // h = h1 + h2 per tap index, h2 rounded with the residual of the previous
-    // taps carried along (error diffusion)
-    std::vector<unsigned short> t1(2 * size_t(ns)), t2(2 * size_t(ns));
+    // NP = 2: h = h1 + h2 per tap index, h2 rounded with the residual of the
+    // previous taps carried along (error diffusion).  NP = 3: h = h1 + h2 + h3,
+    // each the nearest bf16 of what the previous parts leave (exact)
+    std::vector<unsigned short> t1(2 * size_t(ns)), t2(2 * size_t(ns)), t3(2 * size_t(ns), 0);
     double carry = 0.0;
     for (int idx = 0; idx < 2 * ns; idx++) {
         const double h = double(h_taps[idx]);
         t1[idx] = host_bf16_rn(float(h));
-        const double want = (h - host_bf16_f(t1[idx])) + carry;
-        t2[idx] = host_bf16_rn(float(want));
-        carry = want - host_bf16_f(t2[idx]);
+        if (NP == 2) {
+            const double want = (h - host_bf16_f(t1[idx])) + carry;
+            t2[idx] = host_bf16_rn(float(want));
+            carry = want - host_bf16_f(t2[idx]);
+        } else {
+            const double r1 = h - host_bf16_f(t1[idx]);
+            t2[idx] = host_bf16_rn(float(r1));
+            t3[idx] = host_bf16_rn(float(r1 - host_bf16_f(t2[idx])));
+        }
     }
-    std::vector<unsigned short> frag(2 * size_t(stride));
+    std::vector<unsigned short> frag(NP * size_t(stride));
     for (int di = 0; di < n_diag; di++) {
         const int d = di * 16 - (kp - kTK);
         for (int lane = 0; lane < 64; lane++)
@@ -245,6 +280,7 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
                 const size_t o = (size_t(di) * 64 + lane) * 8 + j;
                 frag[o] = in ? t1[idx] : 0;
                 frag[stride + o] = in ? t2[idx] : 0;
+                if (NP == 3) frag[2 * stride + o] = in ? t3[idx] : 0;
             }
     }
     void *dt = nullptr;
@@ -269,25 +305,34 @@ int launch_filter_toeplitz_t(lct_ctx *ctx, int ns, int nt, const float *h_taps, 
     // workgroups still cover every column tile)
     while (a.n_n_wg > 8 && (a.n_m_tiles * a.n_n_wg) % 8 != 0) a.n_n_wg--;
     a.xcd_remap = (a.n_m_tiles * a.n_n_wg) % 8 == 0 ? 1 : 0;
-    hipLaunchKernelGGL(toeplitz_filter_kernel<T>, dim3(unsigned(a.n_m_tiles * a.n_n_wg)), dim3(256),
-                       0, ctx->stream, a);
+    hipLaunchKernelGGL((toeplitz_filter_kernel<T, NP>), dim3(unsigned(a.n_m_tiles * a.n_n_wg)),
+                       dim3(256), 0, ctx->stream, a);
     LCT_CHECK_HIP(hipGetLastError());
     return LCT_OK;
 }
 
-int launch_filter_toeplitz(lct_ctx *ctx, int storage, int ns, int nt, const float *h_taps,
+template <int NP>
+static int launch_filter_toeplitz_np(lct_ctx *ctx, int storage, int ns, int nt, const float *h_taps,
+                                     const float *d_in, void *d_q, long n_cols, int cp, int rp)
+{
+    if (storage == LCT_F32)
+        return launch_filter_toeplitz_t<float, NP>(ctx, ns, nt, h_taps, d_in, static_cast<float *>(d_q),
+                                                   n_cols, cp, rp);
+    if (storage == LCT_F16)
+        return launch_filter_toeplitz_t<__half, NP>(ctx, ns, nt, h_taps, d_in,
+                                                    static_cast<__half *>(d_q), n_cols, cp, rp);
+    return launch_filter_toeplitz_t<__hip_bfloat16, NP>(ctx, ns, nt, h_taps, d_in,
+                                                        static_cast<__hip_bfloat16 *>(d_q), n_cols, cp, rp);
+}
+
+int launch_filter_toeplitz(lct_ctx *ctx, int algo, int storage, int ns, int nt, const float *h_taps,
                            const float *d_in, void *d_q, long n_cols, int cp, int rp)
 {
     if (ns % 4 != 0)
         return set_error(LCT_ERR_UNSUPPORTED, "toeplitz filter: ns must be a multiple of 4");
-    if (storage == LCT_F32)
-        return launch_filter_toeplitz_t<float>(ctx, ns, nt, h_taps, d_in, static_cast<float *>(d_q),
-                                               n_cols, cp, rp);
-    if (storage == LCT_F16)
-        return launch_filter_toeplitz_t<__half>(ctx, ns, nt, h_taps, d_in, static_cast<__half *>(d_q),
-                                                n_cols, cp, rp);
-    return launch_filter_toeplitz_t<__hip_bfloat16>(ctx, ns, nt, h_taps, d_in,
-                                                    static_cast<__hip_bfloat16 *>(d_q), n_cols, cp, rp);
+    if (algo == LCT_FILTER_TOEPLITZ_BF16X6)
+        return launch_filter_toeplitz_np<3>(ctx, storage, ns, nt, h_taps, d_in, d_q, n_cols, cp, rp);
+    return launch_filter_toeplitz_np<2>(ctx, storage, ns, nt, h_taps, d_in, d_q, n_cols, cp, rp);
 }
 
 }  // namespace lct

@@ -127,7 +127,9 @@ class Options:
 def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
     """Named paths: "plain" is the reference's per-block FP32 loop on the GPU;
     "fast_f32" / "mixed_f16" / "mixed_bf16" are the HBM-resident paths with
-    f32 / f16 / bf16 filtered-projection storage (FP32 accumulation)."""
+    f32 / f16 / bf16 filtered-projection storage (FP32 accumulation);
+    "fast_f32_x6" is "fast_f32" with the six-product BF16 Toeplitz MFMA filter
+    (FP32-level accuracy) instead of the FP32 FMA one."""
     if name == "plain":
         return Options(window=window)
     if name == "fast_f32":
@@ -140,6 +142,9 @@ def preset(name: str, window: int = _lib.WINDOW_NONE) -> Options:
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)
     if name == "resident_direct_f16":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window, storage=_lib.F16)
+    if name == "fast_f32_x6":
+        return Options(pipeline=_lib.PIPE_RESIDENT, window=window,
+                       filter_algo=_lib.FILTER_TOEPLITZ_BF16X6)
     if name == "fast_f32_toeplitz":
         return Options(pipeline=_lib.PIPE_RESIDENT, window=window,
                        filter_algo=_lib.FILTER_TOEPLITZ_BF16X3)

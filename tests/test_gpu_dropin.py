@@ -30,6 +30,7 @@ def case(tmp_path_factory):
 @pytest.mark.parametrize("recon_type,kind", [
     ("Helicalfdkramprebingpu-hann", "fp32"),
     ("Helicalfdkramprebingpu-hann-resident", "fp32"),
+    ("Helicalfdkramprebingpu-hann-resident-x6", "fp32"),
     ("Helicalfdkramprebingpu-hann-resident-f16", "mixed"),
 ])
 def test_cli_reconstructs_like_oracle(case, recon_type, kind):
@@ -48,6 +49,7 @@ def test_cli_reconstructs_like_oracle(case, recon_type, kind):
 @pytest.mark.parametrize("recon_type,tol", [
     ("Helicalfdkramprebingpu-hann", 2e-6),
     ("Helicalfdkramprebingpu-hann-resident", 2e-6),       # FP32 direct filter into Q
+    ("Helicalfdkramprebingpu-hann-resident-x6", 2e-6),    # BF16x6 MFMA filter, f32 storage
     ("Helicalfdkramprebingpu-hann-resident-f16", 2e-3),   # BF16x3 MFMA filter, f16 storage
 ])
 def test_cli_rebin_raw_is_last_filtered_block(case, recon_type, tol):

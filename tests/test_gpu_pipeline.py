@@ -41,7 +41,8 @@ def cfg1():
 
 
 @pytest.mark.parametrize("preset,tol", [("plain", 1e-4), ("fast_f32", 1e-4),
-                                        ("fast_f32_gather", 1e-4), ("fast_f32_toeplitz", 2e-4)])
+                                        ("fast_f32_gather", 1e-4), ("fast_f32_toeplitz", 2e-4),
+                                        ("fast_f32_x6", 1e-4)])
 def test_fp32_paths_match_oracle(ctx, cfg1, preset, tol):
     w, fan, ref = cfg1
     print(f"{preset} on {w.name}", flush=True)

@@ -83,9 +83,11 @@ def test_filter(ctx, data, name, window):
 
 @pytest.mark.parametrize("name", CASES)
 @pytest.mark.parametrize("algo,storage,tol", [(0, 0, 2e-6), (1, 0, 1e-4), (1, 1, 1e-3),
-                                              (1, 2, 5e-3), (0, 1, 1e-3)])
+                                              (1, 2, 5e-3), (0, 1, 1e-3), (2, 0, 2e-6),
+                                              (2, 1, 1e-3)])
 def test_filter_into_backprojector_layout(ctx, data, name, algo, storage, tol):
-    """Direct and BF16x3 Toeplitz-MFMA filters, Q[view][ns+3][nt+8] output."""
+    """Direct, BF16x3 and BF16x6 Toeplitz-MFMA filters, Q[view][ns+3][nt+8]
+    output; the six-product form holds the direct FP32 filter's tolerance."""
     w, pl, a, c, fan, win, s = data[name]
     g = pl.ctg
     print(f"filter->Q {name} algo={algo} storage={storage}", flush=True)
