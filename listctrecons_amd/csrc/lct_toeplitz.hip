@@ -79,6 +79,7 @@ toeplitz_filter_kernel(ToeplitzArgs A)
     static_assert(NP == 2 || NP == 3, "two or three operand parts");
     __shared__ __attribute__((aligned(16))) unsigned short bh[2 * kTN * kBRow];
     __shared__ __attribute__((aligned(16))) unsigned short bl[2 * kTN * kBRow];
+    // third data part (NP = 3; a 16-byte placeholder, never touched, for NP = 2)
     __shared__ __attribute__((aligned(16))) unsigned short b3[NP == 3 ? 2 * kTN * kBRow : 8];
 
     const int tid = threadIdx.x;
