@@ -174,6 +174,10 @@ toeplitz_filter_kernel(ToeplitzArgs A)
         for (int ks = 0; ks < n_ksteps; ks++) {
             const int buf = ks & 1;
             if (ks + 1 < n_ksteps) stage_b(n0, ks + 1, xb);
+            // the compiler's MFMA / memory interleaving strategy for the k-step (it
+            // only orders instructions): 1.76 -> 1.63 ms (NP = 2, strategy 1) and
+            // 2.93 -> 2.84 ms (NP = 3, strategy 0) on the clinical scan
+            __builtin_amdgcn_iglp_opt(NP == 2 ? 1 : 0);
             bf16x8 n1[2], n2[2], n3[NP == 3 ? 2 : 1];
             load_a(ks + 1 < n_ksteps ? ks + 1 : ks, n1, n2, n3);
 #pragma unroll
